@@ -1,0 +1,12 @@
+"""acumen_amd -- MI355X (gfx950) reconstruction path of Acumen's neutron CT:
+radon / iradon (ramp filter + backprojection) / FBP / split-Bregman TV.
+
+The compute lives in libacumen_hip.so (hand-written HIP kernels behind the C
+ABI of include/acumen.h); this package is its host-side front end.
+"""
+from ._lib import (ACU_IMG_FBP, ACU_IMG_TV, ACU_SINO_F, ACU_TV_BATCHED, ACU_TV_GRAPH,
+                   ACU_TV_PLAIN, AcumenError)
+from .tomography import Batch, Context, Tomography
+
+__all__ = ["Batch", "Context", "Tomography", "AcumenError", "ACU_IMG_TV", "ACU_IMG_FBP",
+           "ACU_SINO_F", "ACU_TV_PLAIN", "ACU_TV_BATCHED", "ACU_TV_GRAPH"]

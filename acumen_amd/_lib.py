@@ -1,0 +1,96 @@
+"""ctypes binding of libacumen_hip.so (declared in include/acumen.h).
+
+The library is the product: there is no CPU fallback.  Loading fails with an
+ImportError-like exception when the shared library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libacumen_hip.so")
+
+ACU_OK = 0
+ACU_ERR_ARG = 1
+ACU_ERR_HIP = 2
+ACU_ERR_STATE = 3
+
+ACU_IMG_TV = 0
+ACU_IMG_FBP = 1
+ACU_SINO_F = 2
+
+ACU_TV_PLAIN = 0
+ACU_TV_BATCHED = 1
+ACU_TV_GRAPH = 2
+
+KIND_RAMP, KIND_BP, KIND_CELLS, KIND_FP = 0, 1, 2, 3
+
+# Every symbol the header declares, with (restype, argtypes).
+_c_int, _c_float, _vp = ctypes.c_int, ctypes.c_float, ctypes.c_void_p
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+SIGNATURES = {
+    "acu_last_error": (ctypes.c_char_p, []),
+    "acu_ctx_create": (_c_int, [_c_int, ctypes.POINTER(_vp)]),
+    "acu_ctx_destroy": (_c_int, [_vp]),
+    "acu_ctx_synchronize": (_c_int, [_vp]),
+    "acu_plan_create": (_c_int, [_vp, _c_int, _c_int, _c_int, _fp, ctypes.POINTER(_vp)]),
+    "acu_batch_create": (_c_int, [_vp, _c_int, _ip, _ip, _ip, ctypes.POINTER(_fp),
+                                  ctypes.POINTER(_vp)]),
+    "acu_batch_destroy": (_c_int, [_vp]),
+    "acu_batch_nslices": (_c_int, [_vp]),
+    "acu_batch_set_theta": (_c_int, [_vp, _c_int, _fp]),
+    "acu_batch_set_sino": (_c_int, [_vp, _c_int, _fp]),
+    "acu_batch_get_image": (_c_int, [_vp, _c_int, _c_int, _fp]),
+    "acu_batch_device_ptr": (_vp, [_vp, _c_int, _c_int]),
+    "acu_batch_fbp": (_c_int, [_vp]),
+    "acu_batch_tvopt": (_c_int, [_vp, _c_float, _c_float, _c_int, _c_int, _c_int, _c_int]),
+    "acu_batch_get_delta": (_c_int, [_vp, _c_int, _fp, _c_int]),
+    "acu_radon_dev": (_c_int, [_vp, _c_int, _vp, _vp]),
+    "acu_iradon_dev": (_c_int, [_vp, _c_int, _vp, _vp]),
+    "acu_ramp_dev": (_c_int, [_vp, _c_int, _vp, _vp]),
+    "acu_backproject_dev": (_c_int, [_vp, _c_int, _vp, _vp]),
+    "acu_radon": (_c_int, [_vp, _fp, _fp]),
+    "acu_iradon": (_c_int, [_vp, _fp, _fp]),
+    "acu_get_slice": (_c_int, [_vp, _fp, _fp]),
+    "acu_tvopt": (_c_int, [_vp, _fp, _c_float, _c_float, _c_int, _c_int, _c_int, _fp, _fp, _fp]),
+    "acu_ctx_set_timing": (_c_int, [_vp, _c_int]),
+    "acu_ctx_kernel_time": (_c_int, [_vp, _c_int, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_longlong)]),
+}
+
+_lib = None
+
+
+class AcumenError(RuntimeError):
+    """A nonzero status from the library, with its message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"acumen error {code}: {msg}")
+        self.code = code
+
+
+def load(path: str | None = None):
+    """Load the library (once).  Raises FileNotFoundError if it is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(
+            f"{p} is missing: build it with `python -m acumen_amd.build` "
+            "(the HIP library has no CPU fallback)")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != ACU_OK:
+        raise AcumenError(rc, load().acu_last_error().decode(errors="replace"))

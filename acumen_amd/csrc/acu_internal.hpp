@@ -1,0 +1,151 @@
+// acu_internal.hpp -- shared declarations of the gfx950 reconstruction library.
+//
+// Layout in HBM (per slice, N = image side = detector bins, T = angles):
+//   image arrays   N*N fp32, x fastest (the reference's img(x,y), column-major)
+//   sinograms      N*T fp32, detector fastest, one row per angle
+//   cells          (N+1)*(N+1) float4, bilinear cells of an image (see fp.hip)
+//   angle tables   T float2 (sin, cos) of the reference's ctheta, computed on
+//                  the host with the reference's FP32 expression
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+// Address-space views for device code: uniform tables read through the
+// constant space become scalar (SMEM) loads, data pointers read through the
+// global space become global_load with an SGPR base.
+#define ACU_GLOBAL __attribute__((address_space(1)))
+#define ACU_CONST __attribute__((address_space(4)))
+
+namespace acu {
+
+template <class T>
+__device__ __forceinline__ const ACU_CONST T* as_const(const T* p) {
+  return (const ACU_CONST T*)p;
+}
+template <class T>
+__device__ __forceinline__ const ACU_GLOBAL T* as_global(const T* p) {
+  return (const ACU_GLOBAL T*)p;
+}
+// float2 element i of a table read through the constant space (HIP's vector
+// types cannot be copied out of a non-generic address space directly)
+__device__ __forceinline__ float2 const_f2(const float2* tab, size_t i) {
+  const ACU_CONST float* p = (const ACU_CONST float*)tab + 2 * i;
+  return make_float2(p[0], p[1]);
+}
+__device__ __forceinline__ float4 global_f4(const ACU_GLOBAL float4* tab, unsigned i) {
+  const ACU_GLOBAL float4* p = tab + i;
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = *(const ACU_GLOBAL v4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// The dispatcher deals workgroups round-robin over the 8 XCDs of MI355X,
+// each with its own L2.  xcd_remap turns the hardware block index of a grid
+// of xcd_grid(n) blocks into a logical index in [0, xcd_grid(n)) such that
+// the blocks one XCD runs at the same time are consecutive logical blocks;
+// indices >= n are padding and exit at once.
+constexpr int XCDS = 8;
+__host__ __device__ inline int xcd_grid(int n) { return (n + XCDS - 1) / XCDS * XCDS; }
+__device__ __forceinline__ int xcd_remap(int b, int grid) {
+  return (b % XCDS) * (grid / XCDS) + b / XCDS;
+}
+
+// ---- backprojection tiling (bp.hip) ----
+constexpr int BP_TX = 64;        // pixels along x per tile (one per lane)
+constexpr int BP_TY = 32;        // pixels along y per tile
+constexpr int BP_THREADS = 256;  // 4 waves; each wave owns BP_TY/4 rows
+constexpr int BP_PY = BP_TY / (BP_THREADS / BP_TX);  // rows per thread
+constexpr int BP_AB = 32;        // angles staged per LDS block
+constexpr int BP_W = 76;         // detector bins staged per angle (>= ceil(hypot(63,31)) + 4)
+
+// ---- forward projection (fp.hip) ----
+constexpr int FP_THREADS = 256;  // rays per workgroup (one angle per workgroup)
+
+// ---- ramp filter GEMM (filter.hip) ----
+constexpr int RF_BM = 64;   // detector rows per tile
+constexpr int RF_BN = 64;   // angles per tile
+constexpr int RF_BK = 32;   // reduction step
+constexpr int RF_THREADS = 256;
+
+// ---- elementwise / reductions (tv.hip) ----
+constexpr int EW_THREADS = 256;
+constexpr int EW_ITEMS = 4;      // elements per thread per block pass
+
+// One slice of a batched operator launch.
+struct BPTask {
+  int N, T;
+  float mid;             // (N+1)/2 with integer division
+  int tile_begin;        // first global tile index of this slice
+  int ntx, nty;          // tiles along x and y
+  const float* sino;     // filtered sinogram (N*T)
+  const float* cmask;    // N*N
+  const float2* sc;      // per-angle (sin, cos) for backprojection
+  const uint8_t* tile_active;  // nty*ntx: 1 if any cmask > 0 in the tile
+  float* img;            // N*N out
+};
+
+struct FPTask {
+  int N, T;
+  float mid;
+  int block_begin;       // first global block index of this slice
+  int nrb;               // ray blocks per angle = ceil(N / FP_THREADS); block = rb * T + angle
+  const float4* cells;   // (N+1)^2 cells, x-major
+  const float4* cellsT;  // (N+1)^2 cells, y-major (transposed)
+  const float2* sc;      // per-angle (sin, cos) for projection
+  float* sino;           // N*T out
+};
+
+struct RFTask {
+  int N, T;
+  int tile_begin;
+  int tm, tn;            // tiles along detector rows and angles
+  const float* h;        // 2N+1 taps
+  const float* sino;     // N*T in
+  float* out;            // N*T out
+};
+
+struct CellTask {
+  int N;
+  int block_begin;
+  const float* img;      // N*N in (optionally updated in place, see tv.hip)
+  float4* cells;
+  float4* cellsT;
+};
+
+// Launchers (stream-ordered, no host synchronisation).  A launch covers the
+// global blocks [base, base+blocks) of a task array: base = 0 and all blocks
+// for a whole batch, or one slice's own range.
+void launch_ramp(const RFTask* d_tasks, int ntasks, int base, int tiles, hipStream_t s);
+void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, hipStream_t s);
+void launch_cells(const CellTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
+void launch_forward(const FPTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
+
+// Host-side helpers.
+int bp_tiles_x(int N);
+int bp_tiles_y(int N);
+int rf_tiles_m(int N);
+int rf_tiles_n(int T);
+int fp_ray_blocks(int N);
+int cell_blocks(int N);
+
+}  // namespace acu
+
+#define ACU_HIP_CHECK(expr)                                                  \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess) {                                                  \
+      throw acu::Error(std::string(#expr) + ": " + hipGetErrorString(_e));   \
+    }                                                                        \
+  } while (0)
+
+namespace acu {
+struct Error {
+  std::string msg;
+  explicit Error(std::string m) : msg(std::move(m)) {}
+};
+}  // namespace acu

@@ -1,0 +1,812 @@
+// api.hip -- host runtime and C ABI (include/acumen.h) of the gfx950
+// reconstruction library: plans/batches (the reference's module state),
+// operator launches, and the split-Bregman TV orchestration.
+#include "acu_tv.hpp"
+#include "acumen.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+thread_local std::string g_err = "no error";
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+}  // namespace
+
+struct acu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  struct Pending { int kind; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[4] = {0, 0, 0, 0};
+  long long count[4] = {0, 0, 0, 0};
+
+  hipEvent_t ev() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e;
+    ACU_HIP_CHECK(hipEventCreate(&e));
+    return e;
+  }
+  // Fold finished timing events into the totals (call after a stream sync).
+  void drain() {
+    for (auto& p : pending) {
+      float t = 0.f;
+      ACU_HIP_CHECK(hipEventElapsedTime(&t, p.a, p.b));
+      ms[p.kind] += t;
+      count[p.kind] += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  void sync() {
+    ACU_HIP_CHECK(hipStreamSynchronize(stream));
+    drain();
+  }
+};
+
+namespace acu {
+namespace {
+
+// ---- host tables: the reference's FP32 expressions, one rounding each ----
+#pragma clang fp contract(off)
+float ref_pi() { return 4.0f * atanf(1.0f); }
+
+// tv_hpc.F90:53-56
+void make_filter(int N, float* h) {
+  const float pi = ref_pi();
+  for (int jx = 1; jx <= 2 * N + 1; ++jx) {
+    const int k = jx - 1 - N;
+    const int q = 4 * k * k - 1;
+    h[jx - 1] = (-(2.0f * (float)N)) / ((pi * pi) * (float)q);
+  }
+}
+
+// tv_hpc.F90:58-74
+void make_cmask(int N, int T, int wpu, float* c) {
+  const float pi = ref_pi();
+  const float rad_n = (float)(N - 1) / 2.0f - (float)wpu;
+  const float half = ((float)N - 1.0f) / 2.0f;
+  for (int jx = 1; jx <= N; ++jx)
+    for (int jy = 1; jy <= N; ++jy) {
+      const float dx = ((float)jx - half) - 1.0f;
+      const float dy = ((float)jy - half) - 1.0f;
+      const float rr = sqrtf(dx * dx + dy * dy);
+      const float rtmp = 1.0f - (2.0f * (rr - rad_n)) / (float)wpu;
+      float v;
+      if (rtmp <= 0.0f) v = 0.0f;
+      else if (rtmp >= 1.0f) v = pi / (float)T;
+      else {
+        const float e1 = expf(1.0f - 1.0f / (1.0f - rtmp));
+        v = (pi * expf(1.0f - 1.0f / (1.0f - e1))) / (float)T;
+      }
+      c[(size_t)(jy - 1) * N + (jx - 1)] = v;
+    }
+}
+
+// ctheta of backprojection (tv_hpc.F90:113) or projection (:142), sin/cos once per angle
+void make_angles(int T, const float* th, bool bp, float2* sc) {
+  const float pi = ref_pi();
+  for (int j = 0; j < T; ++j) {
+    const float t = th[j];
+    const float ct = bp ? (pi * (90.0f - t)) / 180.0f : (pi * (t - 90.0f)) / 180.0f;
+    sc[j] = make_float2(sinf(ct), cosf(ct));
+  }
+}
+#pragma clang fp contract(on)
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Carves typed arrays out of one device allocation (two passes: measure, then place).
+struct Carver {
+  char* base = nullptr;
+  size_t off = 0;
+  template <class T> T* take(size_t n) {
+    off = align_up(off, 256);
+    T* p = reinterpret_cast<T*>(base + off);
+    off += n * sizeof(T);
+    return p;
+  }
+};
+
+template <class T>
+struct DevArray {
+  T* d = nullptr;
+  int n = 0;
+  void upload(const std::vector<T>& h) {
+    if (d) { hipFree(d); d = nullptr; }
+    n = (int)h.size();
+    if (n == 0) return;
+    ACU_HIP_CHECK(hipMalloc(&d, sizeof(T) * h.size()));
+    ACU_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  }
+  void release() { if (d) hipFree(d); d = nullptr; n = 0; }
+};
+
+struct Slice {
+  int N = 0, T = 0, wpu = 0;
+  float mid = 0.f;
+  std::vector<float> theta;
+  bool theta_set = false;
+  int ntx = 0, nty = 0;
+  // operator arena
+  float *h = nullptr, *cmask = nullptr;
+  float2 *scbp = nullptr, *scfp = nullptr;
+  uint8_t* tact = nullptr;
+  float *f = nullptr, *sinoCS = nullptr, *fbp = nullptr, *ruk = nullptr, *rp = nullptr;
+  float4 *cells = nullptr, *cellsT = nullptr;
+  // TV arena
+  float *uk = nullptr, *ukp = nullptr, *dkx = nullptr, *dky = nullptr, *bkx = nullptr,
+        *bky = nullptr, *ifkt = nullptr, *iukt = nullptr, *r = nullptr, *p = nullptr,
+        *irpt = nullptr, *Ap = nullptr, *fk = nullptr;
+  double *scal = nullptr, *partials = nullptr;
+  unsigned* counter = nullptr;
+  float* delta = nullptr;
+  int delta_cap = 0;
+  bool fbp_valid = false, tv_valid = false;
+  // global block ranges in the batched launches
+  int rf_begin = 0, rf_n = 0, bp_begin = 0, bp_n = 0, cl_begin = 0, cl_n = 0,
+      fp_begin = 0, fp_n = 0, ew_begin = 0, ew_n = 0, sw_begin = 0, sw_n = 0;
+};
+
+}  // namespace
+}  // namespace acu
+
+struct acu_batch {
+  acu_ctx* ctx = nullptr;
+  std::vector<acu::Slice> s;
+  void* ops_mem = nullptr;
+  void* tv_mem = nullptr;
+  int tv_delta_cap = 0;
+  int rf_total = 0, bp_total = 0, cl_total = 0, fp_total = 0, ew_total = 0, sw_total = 0;
+  // task sets for the fixed operand combinations of FBP / TV
+  acu::DevArray<acu::RFTask> rf_f, rf_fk, rf_ruk, rf_rp;
+  acu::DevArray<acu::BPTask> bp_fbp, bp_ifkt, bp_iukt, bp_irpt;
+  acu::DevArray<acu::CellTask> cl_uk, cl_p;
+  acu::DevArray<acu::FPTask> fp_ruk, fp_rp;
+  acu::DevArray<acu::TVTask> tv;
+  float tv_mu = 0.f, tv_lambda = 0.f;
+  // single-slot task buffer for calls on caller-provided device pointers
+  void* adhoc = nullptr;
+  // graph of one outer iteration
+  hipGraphExec_t graph = nullptr;
+  int graph_ninner = -1, graph_ncg = -1;
+
+  ~acu_batch() {
+    if (graph) hipGraphExecDestroy(graph);
+    rf_f.release(); rf_fk.release(); rf_ruk.release(); rf_rp.release();
+    bp_fbp.release(); bp_ifkt.release(); bp_iukt.release(); bp_irpt.release();
+    cl_uk.release(); cl_p.release(); fp_ruk.release(); fp_rp.release(); tv.release();
+    if (adhoc) hipFree(adhoc);
+    if (tv_mem) hipFree(tv_mem);
+    if (ops_mem) hipFree(ops_mem);
+  }
+};
+
+namespace acu {
+namespace {
+
+enum Kind { K_RF = 0, K_BP = 1, K_CELLS = 2, K_FP = 3 };
+
+template <class F>
+void timed(acu_ctx* c, int kind, F&& launch) {
+  if (!c->timing) { launch(); return; }
+  hipEvent_t a = c->ev(), b = c->ev();
+  ACU_HIP_CHECK(hipEventRecord(a, c->stream));
+  launch();
+  ACU_HIP_CHECK(hipEventRecord(b, c->stream));
+  c->pending.push_back({kind, a, b});
+}
+
+void check_launch() { ACU_HIP_CHECK(hipGetLastError()); }
+
+// Operator arena of every slice: tables, f, filtered sinogram, FBP image, cells.
+void layout_ops(acu_batch* b, Carver& cv) {
+  for (auto& s : b->s) {
+    const size_t NN = (size_t)s.N * s.N, NT = (size_t)s.N * s.T, P = (size_t)(s.N + 1);
+    s.h = cv.take<float>(2 * s.N + 1);
+    s.cmask = cv.take<float>(NN);
+    s.scbp = cv.take<float2>(s.T);
+    s.scfp = cv.take<float2>(s.T);
+    s.tact = cv.take<uint8_t>((size_t)s.ntx * s.nty);
+    s.f = cv.take<float>(NT);
+    s.sinoCS = cv.take<float>(NT);
+    s.fbp = cv.take<float>(NN);
+    s.ruk = cv.take<float>(NT);
+    s.rp = cv.take<float>(NT);
+    s.cells = cv.take<float4>(P * P);
+    s.cellsT = cv.take<float4>(P * P);
+  }
+}
+
+void layout_tv(acu_batch* b, Carver& cv, int delta_cap) {
+  for (auto& s : b->s) {
+    const size_t NN = (size_t)s.N * s.N, NT = (size_t)s.N * s.T;
+    float** imgs[] = {&s.uk, &s.ukp, &s.dkx, &s.dky, &s.bkx, &s.bky,
+                      &s.ifkt, &s.iukt, &s.r, &s.p, &s.irpt, &s.Ap};
+    for (float** a : imgs) *a = cv.take<float>(NN);
+    s.fk = cv.take<float>(NT);
+    s.scal = cv.take<double>(SCAL_STRIDE);
+    s.partials = cv.take<double>((size_t)s.ew_n);
+    s.counter = cv.take<unsigned>(1);
+    s.delta = cv.take<float>((size_t)delta_cap);
+    s.delta_cap = delta_cap;
+  }
+}
+
+void upload_tables(acu_batch* b, int i) {
+  auto& s = b->s[i];
+  std::vector<float> h(2 * s.N + 1), cm((size_t)s.N * s.N);
+  make_filter(s.N, h.data());
+  make_cmask(s.N, s.T, s.wpu, cm.data());
+  std::vector<uint8_t> act((size_t)s.ntx * s.nty, 0);
+  for (int y = 0; y < s.N; ++y)
+    for (int x = 0; x < s.N; ++x)
+      if (cm[(size_t)y * s.N + x] != 0.0f) act[(size_t)(y / BP_TY) * s.ntx + x / BP_TX] = 1;
+  ACU_HIP_CHECK(hipMemcpy(s.h, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  ACU_HIP_CHECK(hipMemcpy(s.cmask, cm.data(), cm.size() * 4, hipMemcpyHostToDevice));
+  ACU_HIP_CHECK(hipMemcpy(s.tact, act.data(), act.size(), hipMemcpyHostToDevice));
+}
+
+void upload_theta(acu_batch* b, int i) {
+  auto& s = b->s[i];
+  std::vector<float2> bp(s.T), fp(s.T);
+  make_angles(s.T, s.theta.data(), true, bp.data());
+  make_angles(s.T, s.theta.data(), false, fp.data());
+  ACU_HIP_CHECK(hipMemcpy(s.scbp, bp.data(), bp.size() * sizeof(float2), hipMemcpyHostToDevice));
+  ACU_HIP_CHECK(hipMemcpy(s.scfp, fp.data(), fp.size() * sizeof(float2), hipMemcpyHostToDevice));
+  s.theta_set = true;
+}
+
+RFTask rf_task(const Slice& s, const float* in, float* out) {
+  return RFTask{s.N, s.T, s.rf_begin, rf_tiles_m(s.N), rf_tiles_n(s.T), s.h, in, out};
+}
+BPTask bp_task(const Slice& s, const float* in, float* out) {
+  return BPTask{s.N, s.T, s.mid, s.bp_begin, s.ntx, s.nty, in, s.cmask, s.scbp, s.tact, out};
+}
+CellTask cell_task(const Slice& s, const float* img) {
+  return CellTask{s.N, s.cl_begin, img, s.cells, s.cellsT};
+}
+FPTask fp_task(const Slice& s, float* out) {
+  return FPTask{s.N, s.T, s.mid, s.fp_begin, fp_ray_blocks(s.N), s.cells, s.cellsT, s.scfp, out};
+}
+
+template <class T, class F>
+void build(acu_batch* b, DevArray<T>& arr, F&& make) {
+  std::vector<T> v;
+  for (auto& s : b->s) v.push_back(make(s));
+  arr.upload(v);
+}
+
+void build_op_tasks(acu_batch* b) {
+  build(b, b->rf_f, [](Slice& s) { return rf_task(s, s.f, s.sinoCS); });
+  build(b, b->bp_fbp, [](Slice& s) { return bp_task(s, s.sinoCS, s.fbp); });
+}
+
+void build_tv_tasks(acu_batch* b) {
+  build(b, b->rf_fk, [](Slice& s) { return rf_task(s, s.fk, s.sinoCS); });
+  build(b, b->rf_ruk, [](Slice& s) { return rf_task(s, s.ruk, s.sinoCS); });
+  build(b, b->rf_rp, [](Slice& s) { return rf_task(s, s.rp, s.sinoCS); });
+  build(b, b->bp_ifkt, [](Slice& s) { return bp_task(s, s.sinoCS, s.ifkt); });
+  build(b, b->bp_iukt, [](Slice& s) { return bp_task(s, s.sinoCS, s.iukt); });
+  build(b, b->bp_irpt, [](Slice& s) { return bp_task(s, s.sinoCS, s.irpt); });
+  build(b, b->cl_uk, [](Slice& s) { return cell_task(s, s.uk); });
+  build(b, b->cl_p, [](Slice& s) { return cell_task(s, s.p); });
+  build(b, b->fp_ruk, [](Slice& s) { return fp_task(s, s.ruk); });
+  build(b, b->fp_rp, [](Slice& s) { return fp_task(s, s.rp); });
+}
+
+void build_tv_task_array(acu_batch* b) {
+  build(b, b->tv, [b](Slice& s) {
+    TVTask t{};
+    t.N = s.N; t.T = s.T;
+    t.eb_begin = s.ew_begin; t.nb = s.ew_n;
+    t.sb_begin = s.sw_begin; t.nsb = s.sw_n;
+    t.mu = b->tv_mu; t.lambda = b->tv_lambda;
+    t.uk = s.uk; t.ukp = s.ukp; t.dkx = s.dkx; t.dky = s.dky; t.bkx = s.bkx; t.bky = s.bky;
+    t.ifkt = s.ifkt; t.iukt = s.iukt; t.r = s.r; t.p = s.p; t.irpt = s.irpt; t.Ap = s.Ap;
+    t.f = s.f; t.fk = s.fk; t.ruk = s.ruk;
+    t.scal = s.scal; t.partials = s.partials; t.counter = s.counter;
+    t.delta_log = s.delta; t.delta_cap = s.delta_cap;
+    return t;
+  });
+}
+
+void ensure_tv(acu_batch* b, int delta_cap) {
+  if (b->tv_mem && delta_cap <= b->tv_delta_cap) return;
+  if (b->tv_mem) { hipFree(b->tv_mem); b->tv_mem = nullptr; }
+  if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+  Carver dry;
+  layout_tv(b, dry, delta_cap);
+  ACU_HIP_CHECK(hipMalloc(&b->tv_mem, dry.off + 256));
+  Carver cv{static_cast<char*>(b->tv_mem), 0};
+  layout_tv(b, cv, delta_cap);
+  for (auto& s : b->s) {
+    ACU_HIP_CHECK(hipMemset(s.counter, 0, sizeof(unsigned)));
+    ACU_HIP_CHECK(hipMemset(s.scal, 0, SCAL_STRIDE * sizeof(double)));
+  }
+  b->tv_delta_cap = delta_cap;
+  build_tv_tasks(b);
+}
+
+void require_theta(acu_batch* b) {
+  for (size_t i = 0; i < b->s.size(); ++i)
+    if (!b->s[i].theta_set) throw Error("theta of slice " + std::to_string(i) + " was never set");
+}
+
+// ---- operator sequences (all slices at once, or one slice: sl >= 0) ----
+struct Range { int base, n; };
+Range rf_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->rf_total} : Range{b->s[sl].rf_begin, b->s[sl].rf_n}; }
+Range bp_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->bp_total} : Range{b->s[sl].bp_begin, b->s[sl].bp_n}; }
+Range cl_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->cl_total} : Range{b->s[sl].cl_begin, b->s[sl].cl_n}; }
+Range fp_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->fp_total} : Range{b->s[sl].fp_begin, b->s[sl].fp_n}; }
+Range ew_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->ew_total} : Range{b->s[sl].ew_begin, b->s[sl].ew_n}; }
+Range sw_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->sw_total} : Range{b->s[sl].sw_begin, b->s[sl].sw_n}; }
+
+void iradon_seq(acu_batch* b, const DevArray<RFTask>& rf, const DevArray<BPTask>& bp, int sl) {
+  acu_ctx* c = b->ctx;
+  const Range r1 = rf_range(b, sl), r2 = bp_range(b, sl);
+  timed(c, K_RF, [&] { launch_ramp(rf.d, rf.n, r1.base, r1.n, c->stream); });
+  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, c->stream); });
+  check_launch();
+}
+
+void radon_seq(acu_batch* b, const DevArray<CellTask>& cl, const DevArray<FPTask>& fp, int sl) {
+  acu_ctx* c = b->ctx;
+  const Range r1 = cl_range(b, sl), r2 = fp_range(b, sl);
+  timed(c, K_CELLS, [&] { launch_cells(cl.d, cl.n, r1.base, r1.n, c->stream); });
+  timed(c, K_FP, [&] { launch_forward(fp.d, fp.n, r2.base, r2.n, c->stream); });
+  check_launch();
+}
+
+double read_scalar(acu_batch* b, int sl, int idx) {
+  double v = 0.0;
+  ACU_HIP_CHECK(hipMemcpyAsync(&v, b->s[sl].scal + idx, sizeof(double), hipMemcpyDeviceToHost,
+                               b->ctx->stream));
+  ACU_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+  return v;
+}
+
+// One outer iteration (tv_hpc.F90:205-353) for all slices or one slice.
+// reuse_ruk: ruk already holds radon(uk) for the current uk (true after the
+// prologue and after every add-back, tv_hpc.F90:339, when ninner == 1 or for
+// the first inner iteration), so the radon of :228 is the same result.
+// host_sync: return every reduction to the host before the next kernel, as
+// the reference does (the plain per-slice path).
+void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool reuse_ruk, bool host_sync) {
+  acu_ctx* c = b->ctx;
+  const Range ew = ew_range(b, sl), sw = sw_range(b, sl);
+  const TVTask* tv = b->tv.d;
+  const int nt = b->tv.n;
+  for (int ji = 0; ji < ninner; ++ji) {
+    iradon_seq(b, b->rf_fk, b->bp_ifkt, sl);                       // :215
+    if (!(reuse_ruk && ji == 0)) radon_seq(b, b->cl_uk, b->fp_ruk, sl);  // :228
+    iradon_seq(b, b->rf_ruk, b->bp_iukt, sl);                      // :229
+    launch_tv_residual(tv, nt, ew.base, ew.n, c->stream);          // :207-249
+    check_launch();
+    if (host_sync) (void)read_scalar(b, sl, 0);
+    for (int j = 0; j < ncg; ++j) {                                // :252-303
+      radon_seq(b, b->cl_p, b->fp_rp, sl);
+      iradon_seq(b, b->rf_rp, b->bp_irpt, sl);
+      launch_tv_apdot(tv, nt, ew.base, ew.n, j, c->stream);
+      check_launch();
+      if (host_sync) (void)read_scalar(b, sl, 64 + j);
+      launch_tv_axpy(tv, nt, ew.base, ew.n, j, c->stream);
+      check_launch();
+      if (host_sync) (void)read_scalar(b, sl, j + 1);
+      launch_tv_pupdate(tv, nt, ew.base, ew.n, j, c->stream);
+      check_launch();
+    }
+    launch_tv_shrink(tv, nt, ew.base, ew.n, c->stream);           // :305-333
+    check_launch();
+    if (host_sync) (void)read_scalar(b, sl, 127);
+  }
+  radon_seq(b, b->cl_uk, b->fp_ruk, sl);                           // :339
+  launch_tv_addback(tv, nt, sw.base, sw.n, c->stream);             // :341-346
+  check_launch();
+}
+
+void fbp_all(acu_batch* b, int sl) {
+  iradon_seq(b, b->rf_f, b->bp_fbp, sl);
+}
+
+void tvopt_run(acu_batch* b, float mu, float lambda, int nouter, int ninner, int ncg, int mode) {
+  acu_ctx* c = b->ctx;
+  require_theta(b);
+  if (ncg < 0 || ncg > 60 || nouter < 0 || ninner < 0) throw Error("ncg must be in [0,60], nouter and ninner >= 0");
+  ensure_tv(b, nouter > b->tv_delta_cap ? nouter : b->tv_delta_cap);
+  if (mu != b->tv_mu || lambda != b->tv_lambda || b->tv.n == 0) {
+    b->tv_mu = mu;
+    b->tv_lambda = lambda;
+    build_tv_task_array(b);
+    if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+  }
+  // prologue (tv_hpc.F90:176-202): FBP warm start, uk = fbp, d = b = 0, fk = f
+  fbp_all(b, -1);
+  for (auto& s : b->s)
+    ACU_HIP_CHECK(hipMemcpyAsync(s.uk, s.fbp, (size_t)s.N * s.N * 4, hipMemcpyDeviceToDevice, c->stream));
+  const Range ew = ew_range(b, -1), sw = sw_range(b, -1);
+  launch_tv_init(b->tv.d, b->tv.n, ew.base, ew.n, sw.base, sw.n, c->stream);
+  check_launch();
+  for (auto& s : b->s) { s.fbp_valid = true; s.tv_valid = false; }
+  if (mode == ACU_TV_PLAIN) {
+    for (int sl = 0; sl < (int)b->s.size(); ++sl)
+      for (int jo = 0; jo < nouter; ++jo) outer_body(b, sl, ninner, ncg, false, true);
+  } else {
+    if (nouter > 0) radon_seq(b, b->cl_uk, b->fp_ruk, -1);  // ruk = radon(uk) for :228
+    const bool graph = (mode == ACU_TV_GRAPH) && !c->timing;
+    if (!graph) {
+      for (int jo = 0; jo < nouter; ++jo) outer_body(b, -1, ninner, ncg, true, false);
+    } else if (nouter > 0) {
+      if (!b->graph || b->graph_ninner != ninner || b->graph_ncg != ncg) {
+        if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+        hipGraph_t g;
+        ACU_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+          outer_body(b, -1, ninner, ncg, true, false);
+        } catch (...) {
+          hipStreamEndCapture(c->stream, &g);
+          throw;
+        }
+        ACU_HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+        ACU_HIP_CHECK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+        ACU_HIP_CHECK(hipGraphDestroy(g));
+        b->graph_ninner = ninner;
+        b->graph_ncg = ncg;
+      }
+      for (int jo = 0; jo < nouter; ++jo) ACU_HIP_CHECK(hipGraphLaunch(b->graph, c->stream));
+    }
+  }
+  for (auto& s : b->s) s.tv_valid = true;
+}
+
+void adhoc_slot(acu_batch* b, size_t bytes) {
+  if (!b->adhoc) ACU_HIP_CHECK(hipMalloc(&b->adhoc, 256));
+  if (bytes > 256) throw Error("task too large");
+  // the previous call on caller pointers may still read the slot
+  ACU_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+}
+
+template <class T>
+T* put_adhoc(acu_batch* b, const T& t) {
+  adhoc_slot(b, sizeof(T));
+  ACU_HIP_CHECK(hipMemcpy(b->adhoc, &t, sizeof(T), hipMemcpyHostToDevice));
+  return static_cast<T*>(b->adhoc);
+}
+
+Slice& slice_of(acu_batch* b, int sl) {
+  if (sl < 0 || sl >= (int)b->s.size()) throw Error("slice index out of range");
+  if (!b->s[sl].theta_set) throw Error("theta of slice " + std::to_string(sl) + " was never set");
+  return b->s[sl];
+}
+
+void radon_dev(acu_batch* b, int sl, float* sino, const float* img) {
+  Slice& s = slice_of(b, sl);
+  acu_ctx* c = b->ctx;
+  {
+    CellTask t = cell_task(s, img);
+    t.block_begin = 0;
+    CellTask* d = put_adhoc(b, t);
+    timed(c, K_CELLS, [&] { launch_cells(d, 1, 0, s.cl_n, c->stream); });
+  }
+  {
+    FPTask t = fp_task(s, sino);
+    t.block_begin = 0;
+    FPTask* d = put_adhoc(b, t);
+    timed(c, K_FP, [&] { launch_forward(d, 1, 0, s.fp_n, c->stream); });
+  }
+  check_launch();
+}
+
+void ramp_dev(acu_batch* b, int sl, float* out, const float* in) {
+  Slice& s = slice_of(b, sl);
+  acu_ctx* c = b->ctx;
+  RFTask t = rf_task(s, in, out);
+  t.tile_begin = 0;
+  RFTask* d = put_adhoc(b, t);
+  timed(c, K_RF, [&] { launch_ramp(d, 1, 0, s.rf_n, c->stream); });
+  check_launch();
+}
+
+void bp_dev(acu_batch* b, int sl, float* img, const float* filt) {
+  Slice& s = slice_of(b, sl);
+  acu_ctx* c = b->ctx;
+  BPTask t = bp_task(s, filt, img);
+  t.tile_begin = 0;
+  BPTask* d = put_adhoc(b, t);
+  timed(c, K_BP, [&] { launch_backproject(d, 1, 0, s.bp_n, c->stream); });
+  check_launch();
+}
+
+}  // namespace
+}  // namespace acu
+
+using acu::Error;
+
+#define ACU_TRY(...)                                                         \
+  try {                                                                      \
+    __VA_ARGS__;                                                             \
+    return ACU_OK;                                                           \
+  } catch (const Error& e) {                                                 \
+    return fail(ACU_ERR_HIP, e.msg);                                         \
+  } catch (const std::bad_alloc&) {                                          \
+    return fail(ACU_ERR_HIP, "host allocation failed");                      \
+  }
+
+extern "C" {
+
+const char* acu_last_error(void) { return g_err.c_str(); }
+
+int acu_ctx_create(int device, acu_ctx** out) {
+  if (!out) return fail(ACU_ERR_ARG, "acu_ctx_create: out is NULL");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(ACU_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return fail(ACU_ERR_ARG, "acu_ctx_create: device index out of range");
+  auto* c = new acu_ctx();
+  c->device = device;
+  try {
+    ACU_HIP_CHECK(hipSetDevice(device));
+    ACU_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  } catch (const Error& e) {
+    delete c;
+    return fail(ACU_ERR_HIP, e.msg);
+  }
+  *out = c;
+  return ACU_OK;
+}
+
+int acu_ctx_destroy(acu_ctx* ctx) {
+  if (!ctx) return ACU_OK;
+  hipStreamSynchronize(ctx->stream);
+  for (auto& p : ctx->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+  for (auto e : ctx->pool) hipEventDestroy(e);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return ACU_OK;
+}
+
+int acu_ctx_synchronize(acu_ctx* ctx) {
+  if (!ctx) return fail(ACU_ERR_ARG, "null context");
+  ACU_TRY(hipSetDevice(ctx->device); ctx->sync());
+}
+
+int acu_ctx_set_timing(acu_ctx* ctx, int enable) {
+  if (!ctx) return fail(ACU_ERR_ARG, "null context");
+  ACU_TRY(ctx->sync(); ctx->timing = enable != 0;
+          for (int k = 0; k < 4; ++k) { ctx->ms[k] = 0; ctx->count[k] = 0; });
+}
+
+int acu_ctx_kernel_time(acu_ctx* ctx, int kind, double* total_ms, long long* launches) {
+  if (!ctx || kind < 0 || kind > 3) return fail(ACU_ERR_ARG, "acu_ctx_kernel_time: bad argument");
+  ACU_TRY(ctx->sync(); if (total_ms) *total_ms = ctx->ms[kind];
+          if (launches) *launches = ctx->count[kind]);
+}
+
+int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* ntheta,
+                     const int* wpu, const float* const* theta_deg, acu_batch** out) {
+  if (!ctx || !out || nslices <= 0 || !nimage || !ntheta || !wpu)
+    return fail(ACU_ERR_ARG, "acu_batch_create: bad argument");
+  *out = nullptr;
+  for (int i = 0; i < nslices; ++i) {
+    if (nimage[i] < 3 || nimage[i] > 16000) return fail(ACU_ERR_ARG, "nimage must be in [3, 16000]");
+    if (ntheta[i] < 1 || ntheta[i] > 1 << 20) return fail(ACU_ERR_ARG, "ntheta must be in [1, 2^20]");
+    if (wpu[i] < 1) return fail(ACU_ERR_ARG, "wpu must be >= 1 (the mask divides by it)");
+  }
+  std::unique_ptr<acu_batch> b(new acu_batch());
+  b->ctx = ctx;
+  try {
+    ACU_HIP_CHECK(hipSetDevice(ctx->device));
+    b->s.resize(nslices);
+    int rf = 0, bp = 0, cl = 0, fp = 0, ew = 0, sw = 0;
+    for (int i = 0; i < nslices; ++i) {
+      auto& s = b->s[i];
+      s.N = nimage[i]; s.T = ntheta[i]; s.wpu = wpu[i];
+      s.mid = (float)((s.N + 1) / 2);
+      s.ntx = acu::bp_tiles_x(s.N); s.nty = acu::bp_tiles_y(s.N);
+      s.theta.assign(s.T, 0.0f);
+      s.rf_begin = rf; s.rf_n = acu::rf_tiles_m(s.N) * acu::rf_tiles_n(s.T); rf += s.rf_n;
+      s.bp_begin = bp; s.bp_n = s.ntx * s.nty; bp += s.bp_n;
+      s.cl_begin = cl; s.cl_n = acu::cell_blocks(s.N); cl += s.cl_n;
+      s.fp_begin = fp; s.fp_n = acu::fp_ray_blocks(s.N) * s.T; fp += s.fp_n;
+      s.ew_begin = ew; s.ew_n = acu::ew_blocks((size_t)s.N * s.N); ew += s.ew_n;
+      s.sw_begin = sw; s.sw_n = acu::ew_blocks((size_t)s.N * s.T); sw += s.sw_n;
+    }
+    b->rf_total = rf; b->bp_total = bp; b->cl_total = cl; b->fp_total = fp;
+    b->ew_total = ew; b->sw_total = sw;
+    acu::Carver dry;
+    acu::layout_ops(b.get(), dry);
+    ACU_HIP_CHECK(hipMalloc(&b->ops_mem, dry.off + 256));
+    acu::Carver cv{static_cast<char*>(b->ops_mem), 0};
+    acu::layout_ops(b.get(), cv);
+    for (int i = 0; i < nslices; ++i) {
+      acu::upload_tables(b.get(), i);
+      ACU_HIP_CHECK(hipMemset(b->s[i].f, 0, (size_t)b->s[i].N * b->s[i].T * 4));
+      if (theta_deg && theta_deg[i]) {
+        std::memcpy(b->s[i].theta.data(), theta_deg[i], sizeof(float) * b->s[i].T);
+        acu::upload_theta(b.get(), i);
+      }
+    }
+    acu::build_op_tasks(b.get());
+  } catch (const Error& e) {
+    return fail(ACU_ERR_HIP, e.msg);
+  }
+  *out = b.release();
+  return ACU_OK;
+}
+
+int acu_plan_create(acu_ctx* ctx, int nimage, int ntheta, int wpu, const float* theta_deg,
+                    acu_batch** out) {
+  const float* th[1] = {theta_deg};
+  return acu_batch_create(ctx, 1, &nimage, &ntheta, &wpu, th, out);
+}
+
+int acu_batch_destroy(acu_batch* b) {
+  if (!b) return ACU_OK;
+  hipSetDevice(b->ctx->device);
+  hipStreamSynchronize(b->ctx->stream);
+  delete b;
+  return ACU_OK;
+}
+
+int acu_batch_nslices(const acu_batch* b) { return b ? (int)b->s.size() : 0; }
+
+int acu_batch_set_theta(acu_batch* b, int slice, const float* theta_deg) {
+  if (!b || !theta_deg || slice < 0 || slice >= (int)b->s.size())
+    return fail(ACU_ERR_ARG, "acu_batch_set_theta: bad argument");
+  ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync();
+          std::memcpy(b->s[slice].theta.data(), theta_deg, sizeof(float) * b->s[slice].T);
+          acu::upload_theta(b, slice));
+}
+
+int acu_batch_set_sino(acu_batch* b, int slice, const float* f_host) {
+  if (!b || !f_host || slice < 0 || slice >= (int)b->s.size())
+    return fail(ACU_ERR_ARG, "acu_batch_set_sino: bad argument");
+  auto& s = b->s[slice];
+  ACU_TRY(hipSetDevice(b->ctx->device);
+          ACU_HIP_CHECK(hipMemcpyAsync(s.f, f_host, (size_t)s.N * s.T * 4, hipMemcpyHostToDevice,
+                                       b->ctx->stream));
+          b->ctx->sync());
+}
+
+int acu_batch_get_image(acu_batch* b, int slice, int which, float* img_host) {
+  if (!b || !img_host || slice < 0 || slice >= (int)b->s.size())
+    return fail(ACU_ERR_ARG, "acu_batch_get_image: bad argument");
+  auto& s = b->s[slice];
+  const float* src = nullptr;
+  if (which == ACU_IMG_FBP) {
+    if (!s.fbp_valid) return fail(ACU_ERR_STATE, "no FBP image computed yet");
+    src = s.fbp;
+  } else if (which == ACU_IMG_TV) {
+    if (!s.tv_valid) return fail(ACU_ERR_STATE, "no TV image computed yet");
+    src = s.uk;
+  } else {
+    return fail(ACU_ERR_ARG, "acu_batch_get_image: which must be ACU_IMG_TV or ACU_IMG_FBP");
+  }
+  ACU_TRY(hipSetDevice(b->ctx->device);
+          ACU_HIP_CHECK(hipMemcpyAsync(img_host, src, (size_t)s.N * s.N * 4, hipMemcpyDeviceToHost,
+                                       b->ctx->stream));
+          b->ctx->sync());
+}
+
+float* acu_batch_device_ptr(acu_batch* b, int slice, int which) {
+  if (!b || slice < 0 || slice >= (int)b->s.size()) { fail(ACU_ERR_ARG, "bad slice"); return nullptr; }
+  auto& s = b->s[slice];
+  if (which == ACU_SINO_F) return s.f;
+  if (which == ACU_IMG_FBP) return s.fbp;
+  if (which == ACU_IMG_TV) {
+    try { acu::ensure_tv(b, b->tv_delta_cap > 0 ? b->tv_delta_cap : 1); }
+    catch (const Error& e) { fail(ACU_ERR_HIP, e.msg); return nullptr; }
+    return s.uk;
+  }
+  fail(ACU_ERR_ARG, "bad selector");
+  return nullptr;
+}
+
+int acu_batch_fbp(acu_batch* b) {
+  if (!b) return fail(ACU_ERR_ARG, "null batch");
+  ACU_TRY(hipSetDevice(b->ctx->device); acu::require_theta(b); acu::fbp_all(b, -1);
+          for (auto& s : b->s) s.fbp_valid = true);
+}
+
+int acu_batch_tvopt(acu_batch* b, float mu, float lambda, int nouter, int ninner, int ncg, int mode) {
+  if (!b) return fail(ACU_ERR_ARG, "null batch");
+  if (mode != ACU_TV_PLAIN && mode != ACU_TV_BATCHED && mode != ACU_TV_GRAPH)
+    return fail(ACU_ERR_ARG, "unknown tvopt mode");
+  if (lambda == 0.0f) return fail(ACU_ERR_ARG, "lambda must be nonzero (the shrink threshold is 1/lambda)");
+  ACU_TRY(hipSetDevice(b->ctx->device); acu::tvopt_run(b, mu, lambda, nouter, ninner, ncg, mode));
+}
+
+int acu_batch_get_delta(acu_batch* b, int slice, float* delta_host, int n) {
+  if (!b || !delta_host || slice < 0 || slice >= (int)b->s.size() || n < 0)
+    return fail(ACU_ERR_ARG, "acu_batch_get_delta: bad argument");
+  auto& s = b->s[slice];
+  if (!s.delta || n > s.delta_cap) return fail(ACU_ERR_STATE, "no delta log of that length");
+  ACU_TRY(ACU_HIP_CHECK(hipMemcpyAsync(delta_host, s.delta, (size_t)n * 4, hipMemcpyDeviceToHost,
+                                       b->ctx->stream));
+          b->ctx->sync());
+}
+
+int acu_radon_dev(acu_batch* b, int slice, float* sino_dev, const float* img_dev) {
+  if (!b || !sino_dev || !img_dev) return fail(ACU_ERR_ARG, "acu_radon_dev: null argument");
+  ACU_TRY(hipSetDevice(b->ctx->device); acu::radon_dev(b, slice, sino_dev, img_dev));
+}
+
+int acu_ramp_dev(acu_batch* b, int slice, float* out_dev, const float* sino_dev) {
+  if (!b || !out_dev || !sino_dev) return fail(ACU_ERR_ARG, "acu_ramp_dev: null argument");
+  ACU_TRY(hipSetDevice(b->ctx->device); acu::ramp_dev(b, slice, out_dev, sino_dev));
+}
+
+int acu_backproject_dev(acu_batch* b, int slice, float* img_dev, const float* filtered_dev) {
+  if (!b || !img_dev || !filtered_dev) return fail(ACU_ERR_ARG, "acu_backproject_dev: null argument");
+  ACU_TRY(hipSetDevice(b->ctx->device); acu::bp_dev(b, slice, img_dev, filtered_dev));
+}
+
+int acu_iradon_dev(acu_batch* b, int slice, float* img_dev, const float* sino_dev) {
+  if (!b || !img_dev || !sino_dev) return fail(ACU_ERR_ARG, "acu_iradon_dev: null argument");
+  ACU_TRY(hipSetDevice(b->ctx->device);
+          acu::Slice& s = acu::slice_of(b, slice);
+          acu::ramp_dev(b, slice, s.sinoCS, sino_dev);
+          acu::bp_dev(b, slice, img_dev, s.sinoCS));
+}
+
+namespace {
+// Host-buffer round trip through device scratch for the single-slice forms.
+struct DevBuf {
+  float* p = nullptr;
+  explicit DevBuf(size_t n) { ACU_HIP_CHECK(hipMalloc(&p, n * 4)); }
+  ~DevBuf() { if (p) hipFree(p); }
+};
+}  // namespace
+
+int acu_radon(acu_batch* b, float* sino_host, const float* img_host) {
+  if (!b || !sino_host || !img_host) return fail(ACU_ERR_ARG, "acu_radon: null argument");
+  ACU_TRY(hipSetDevice(b->ctx->device);
+          acu::Slice& s = acu::slice_of(b, 0);
+          DevBuf img((size_t)s.N * s.N), sino((size_t)s.N * s.T);
+          ACU_HIP_CHECK(hipMemcpy(img.p, img_host, (size_t)s.N * s.N * 4, hipMemcpyHostToDevice));
+          acu::radon_dev(b, 0, sino.p, img.p);
+          b->ctx->sync();
+          ACU_HIP_CHECK(hipMemcpy(sino_host, sino.p, (size_t)s.N * s.T * 4, hipMemcpyDeviceToHost)));
+}
+
+int acu_iradon(acu_batch* b, float* img_host, const float* sino_host) {
+  if (!b || !img_host || !sino_host) return fail(ACU_ERR_ARG, "acu_iradon: null argument");
+  ACU_TRY(hipSetDevice(b->ctx->device);
+          acu::Slice& s = acu::slice_of(b, 0);
+          DevBuf img((size_t)s.N * s.N), sino((size_t)s.N * s.T);
+          ACU_HIP_CHECK(hipMemcpy(sino.p, sino_host, (size_t)s.N * s.T * 4, hipMemcpyHostToDevice));
+          acu::ramp_dev(b, 0, s.sinoCS, sino.p);
+          acu::bp_dev(b, 0, img.p, s.sinoCS);
+          b->ctx->sync();
+          ACU_HIP_CHECK(hipMemcpy(img_host, img.p, (size_t)s.N * s.N * 4, hipMemcpyDeviceToHost)));
+}
+
+int acu_get_slice(acu_batch* b, const float* f_host, float* img_fbp_host) {
+  int rc = acu_batch_set_sino(b, 0, f_host);
+  if (rc) return rc;
+  if ((rc = acu_batch_fbp(b))) return rc;
+  return acu_batch_get_image(b, 0, ACU_IMG_FBP, img_fbp_host);
+}
+
+int acu_tvopt(acu_batch* b, const float* f_host, float mu, float lambda, int nouter, int ninner,
+              int ncg, float* img_tv_host, float* img_fbp_host, float* delta_host) {
+  int rc = acu_batch_set_sino(b, 0, f_host);
+  if (rc) return rc;
+  if ((rc = acu_batch_tvopt(b, mu, lambda, nouter, ninner, ncg, ACU_TV_GRAPH))) return rc;
+  if (img_tv_host && (rc = acu_batch_get_image(b, 0, ACU_IMG_TV, img_tv_host))) return rc;
+  if (img_fbp_host && (rc = acu_batch_get_image(b, 0, ACU_IMG_FBP, img_fbp_host))) return rc;
+  if (delta_host && nouter > 0 && (rc = acu_batch_get_delta(b, 0, delta_host, nouter))) return rc;
+  return ACU_OK;
+}
+
+}  // extern "C"

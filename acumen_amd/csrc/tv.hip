@@ -1,0 +1,305 @@
+// tv.hip -- fused elementwise updates and FP64 reductions of the
+// split-Bregman TV solver (tomography/tv_hpc.F90:162-363) on gfx950.
+//
+// The reference runs every loop nest as its own kernel and returns each FP32
+// reduction to the host (12 host round trips per inner iteration at ncg=5).
+// Here each kernel fuses an update with the reduction that consumes it, the
+// reductions are FP64 (wavefront shuffles, then LDS, then the last block of
+// the grid sums the block partials in a fixed order), and every scalar
+// (rsold, p.Ap, rsnew, alpha, beta, Err_diff) lives in device memory, so an
+// outer iteration needs no host synchronisation and can be replayed from a
+// HIP graph.  All kernels are batched: blocks are flattened over slices.
+//
+// Scalar slots per slice (doubles): rs[j] at j (j = 0..ncg), pAp[j] at
+// 64+j, outer-iteration counter at 126, Err_diff at 127.
+#include "acu_tv.hpp"
+
+namespace acu {
+namespace {
+
+constexpr int NT_ = EW_THREADS;
+constexpr int IT = EW_ITEMS;
+constexpr int PER_BLOCK = NT_ * IT;
+
+__device__ __forceinline__ int find_task_e(const TVTask* tasks, int ntasks, int gb) {
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].eb_begin <= gb) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+__device__ __forceinline__ int find_task_s(const TVTask* tasks, int ntasks, int gb) {
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].sb_begin <= gb) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Block-reduce v, write the block partial, and let the last block of this
+// slice sum all partials in index order into *out.  Deterministic.
+__device__ void reduce_to_scalar(double v, double* partials, unsigned* counter, int nblocks,
+                                 int lblock, double* out) {
+  __shared__ double wsum[NT_ / 64];
+  __shared__ bool last;
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wsum[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT_ / 64; ++w) b += wsum[w];
+    partials[lblock] = b;
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    last = (prev == (unsigned)nblocks - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // fixed-order sum of the partials: thread t sums a contiguous chunk
+  const int chunk = (nblocks + NT_ - 1) / NT_;
+  double s = 0.0;
+  const int b0 = threadIdx.x * chunk;
+  for (int b = b0; b < min(nblocks, b0 + chunk); ++b) s += partials[b];
+  s = wave_sum(s);
+  __syncthreads();
+  if (lane == 0) wsum[wave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT_ / 64; ++w) tot += wsum[w];
+    *out = tot;
+    *counter = 0u;  // ready for the next reduction on this slice
+  }
+}
+
+__device__ __forceinline__ float lap(const float* __restrict__ u, int x, int y, int N) {
+  const int xl = x == 0 ? N - 1 : x - 1, xr = x == N - 1 ? 0 : x + 1;
+  const int yl = y == 0 ? N - 1 : y - 1, yr = y == N - 1 ? 0 : y + 1;
+  const size_t row = (size_t)y * N;
+  float l = 4.0f * u[row + x] - u[row + xl];
+  l = l - u[row + xr];
+  l = l - u[(size_t)yl * N + x];
+  l = l - u[(size_t)yr * N + x];
+  return l;
+}
+
+// K9 + K10 + K11 (tv_hpc.F90:207-249): ukp = uk; rhs from d, b; r = rhs -
+// mu*iukt - lambda*L(uk); p = r; rs[0] = sum r^2.
+__global__ __launch_bounds__(NT_) void residual_kernel(const TVTask* __restrict__ tasks, int ntasks, int base) {
+  const TVTask& tk = tasks[find_task_e(tasks, ntasks, base + blockIdx.x)];
+  const int N = tk.N;
+  const int lb = base + blockIdx.x - tk.eb_begin;
+  const size_t NN = (size_t)N * N;
+  const float mu = tk.mu, lam = tk.lambda;
+  double acc = 0.0;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
+    if (e < NN) {
+      const int y = (int)(e / N), x = (int)(e - (size_t)y * N);
+      const int xn = x == N - 1 ? 0 : x + 1, yn = y == N - 1 ? 0 : y + 1;
+      const size_t ex = (size_t)y * N + xn, ey = (size_t)yn * N + x;
+      float t = tk.dkx[ex] - tk.bkx[ex];
+      t = t - tk.dkx[e];
+      t = t + tk.bkx[e];
+      t = t + tk.dky[ey];
+      t = t - tk.bky[ey];
+      t = t - tk.dky[e];
+      t = t + tk.bky[e];
+      const float rhs = mu * tk.ifkt[e] + lam * t;
+      const float rv = (rhs - mu * tk.iukt[e]) - lam * lap(tk.uk, x, y, N);
+      tk.r[e] = rv;
+      tk.p[e] = rv;
+      tk.ukp[e] = tk.uk[e];
+      acc += (double)rv * (double)rv;
+    }
+  }
+  reduce_to_scalar(acc, tk.partials, tk.counter, tk.nb, lb, tk.scal + 0);
+}
+
+// K12 + K13 (tv_hpc.F90:257-274): Ap = mu*irpt + lambda*L(p); pAp[j] = p.Ap
+__global__ __launch_bounds__(NT_) void apdot_kernel(const TVTask* __restrict__ tasks, int ntasks, int base, int j) {
+  const TVTask& tk = tasks[find_task_e(tasks, ntasks, base + blockIdx.x)];
+  const int N = tk.N;
+  const int lb = base + blockIdx.x - tk.eb_begin;
+  const size_t NN = (size_t)N * N;
+  const float mu = tk.mu, lam = tk.lambda;
+  double acc = 0.0;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
+    if (e < NN) {
+      const int y = (int)(e / N), x = (int)(e - (size_t)y * N);
+      const float a = mu * tk.irpt[e] + lam * lap(tk.p, x, y, N);
+      tk.Ap[e] = a;
+      acc += (double)tk.p[e] * (double)a;
+    }
+  }
+  reduce_to_scalar(acc, tk.partials, tk.counter, tk.nb, lb, tk.scal + 64 + j);
+}
+
+// K14 + K15 (tv_hpc.F90:277-292): alpha = rs[j]/pAp[j] (FP64);
+// uk += alpha p; r -= alpha Ap; rs[j+1] = sum r^2.
+// A zero p.Ap (all-zero residual) gives alpha = 0 instead of the
+// reference's 0/0 = NaN.
+__global__ __launch_bounds__(NT_) void axpy_kernel(const TVTask* __restrict__ tasks, int ntasks, int base, int j) {
+  const TVTask& tk = tasks[find_task_e(tasks, ntasks, base + blockIdx.x)];
+  const int N = tk.N;
+  const int lb = base + blockIdx.x - tk.eb_begin;
+  const size_t NN = (size_t)N * N;
+  const double num = tk.scal[64 + j];
+  const float alpha = num != 0.0 ? (float)(tk.scal[j] / num) : 0.0f;
+  double acc = 0.0;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
+    if (e < NN) {
+      tk.uk[e] = tk.uk[e] + alpha * tk.p[e];
+      const float rv = tk.r[e] - alpha * tk.Ap[e];
+      tk.r[e] = rv;
+      acc += (double)rv * (double)rv;
+    }
+  }
+  reduce_to_scalar(acc, tk.partials, tk.counter, tk.nb, lb, tk.scal + j + 1);
+}
+
+// K16 (tv_hpc.F90:294-299): p = r + (rs[j+1]/rs[j]) p
+__global__ __launch_bounds__(NT_) void pupdate_kernel(const TVTask* __restrict__ tasks, int ntasks, int base, int j) {
+  const TVTask& tk = tasks[find_task_e(tasks, ntasks, base + blockIdx.x)];
+  const int N = tk.N;
+  const int lb = base + blockIdx.x - tk.eb_begin;
+  const size_t NN = (size_t)N * N;
+  const double rs = tk.scal[j];
+  const float beta = rs != 0.0 ? (float)(tk.scal[j + 1] / rs) : 0.0f;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
+    if (e < NN) tk.p[e] = tk.r[e] + beta * tk.p[e];
+  }
+}
+
+__device__ __forceinline__ void shrink(float t, float inv_lambda, float& d, float& b) {
+  if (t == 0.0f) {
+    d = 0.0f;
+  } else {
+    const float at = fabsf(t);
+    const float a = fmaxf(at - inv_lambda, 0.0f);
+    d = (a * t) / at;
+  }
+  b = t - d;
+}
+
+// K17 + K18 (tv_hpc.F90:305-333): anisotropic shrink of D u + b at 1/lambda,
+// Err_diff = sum |uk - ukp| (also appended to the per-slice delta log).
+__global__ __launch_bounds__(NT_) void shrink_kernel(const TVTask* __restrict__ tasks, int ntasks, int base) {
+  const TVTask& tk = tasks[find_task_e(tasks, ntasks, base + blockIdx.x)];
+  const int N = tk.N;
+  const int lb = base + blockIdx.x - tk.eb_begin;
+  const size_t NN = (size_t)N * N;
+  const float inv_lambda = 1.0f / tk.lambda;
+  double acc = 0.0;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
+    if (e < NN) {
+      const int y = (int)(e / N), x = (int)(e - (size_t)y * N);
+      const int xp = x == 0 ? N - 1 : x - 1, yp = y == 0 ? N - 1 : y - 1;
+      const float u = tk.uk[e];
+      float d, b;
+      shrink((tk.uk[(size_t)y * N + xp] - u) + tk.bkx[e], inv_lambda, d, b);
+      tk.dkx[e] = d;
+      tk.bkx[e] = b;
+      shrink((tk.bky[e] + tk.uk[(size_t)yp * N + x]) - u, inv_lambda, d, b);
+      tk.dky[e] = d;
+      tk.bky[e] = b;
+      acc += (double)fabsf(u - tk.ukp[e]);
+    }
+  }
+  reduce_to_scalar(acc, tk.partials, tk.counter, tk.nb, lb, tk.scal + 127);
+}
+
+// K19 (tv_hpc.F90:341-346): fk = (fk + f) - ruk, plus the delta log entry.
+__global__ __launch_bounds__(NT_) void addback_kernel(const TVTask* __restrict__ tasks, int ntasks, int base) {
+  const TVTask& tk = tasks[find_task_s(tasks, ntasks, base + blockIdx.x)];
+  const int lb = base + blockIdx.x - tk.sb_begin;
+  const size_t NT = (size_t)tk.N * tk.T;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
+    if (e < NT) tk.fk[e] = (tk.fk[e] + tk.f[e]) - tk.ruk[e];
+  }
+  if (lb == 0 && threadIdx.x == 0) {
+    const int j = (int)tk.scal[126];
+    if (tk.delta_log != nullptr && j < tk.delta_cap) tk.delta_log[j] = (float)tk.scal[127];
+    tk.scal[126] = (double)(j + 1);
+  }
+}
+
+// tvopt init (tv_hpc.F90:187-202), image part: d = b = 0.
+__global__ __launch_bounds__(NT_) void tvinit_kernel(const TVTask* __restrict__ tasks, int ntasks, int base) {
+  const TVTask& tk = tasks[find_task_e(tasks, ntasks, base + blockIdx.x)];
+  const int lb = base + blockIdx.x - tk.eb_begin;
+  const size_t NN = (size_t)tk.N * tk.N;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
+    if (e < NN) {
+      tk.dkx[e] = 0.0f; tk.dky[e] = 0.0f; tk.bkx[e] = 0.0f; tk.bky[e] = 0.0f;
+    }
+  }
+  if (lb == 0 && threadIdx.x == 0) tk.scal[126] = 0.0;  // outer-iteration counter
+}
+
+// tvopt init, sinogram part: fk = f.
+__global__ __launch_bounds__(NT_) void fkinit_kernel(const TVTask* __restrict__ tasks, int ntasks, int base) {
+  const TVTask& tk = tasks[find_task_s(tasks, ntasks, base + blockIdx.x)];
+  const int lb = base + blockIdx.x - tk.sb_begin;
+  const size_t NT = (size_t)tk.N * tk.T;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
+    if (e < NT) tk.fk[e] = tk.f[e];
+  }
+}
+
+}  // namespace
+
+int ew_blocks(size_t n) { return (int)((n + PER_BLOCK - 1) / PER_BLOCK); }
+
+void launch_tv_residual(const TVTask* t, int n, int base, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base);
+}
+void launch_tv_apdot(const TVTask* t, int n, int base, int blocks, int j, hipStream_t s) {
+  hipLaunchKernelGGL(apdot_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base, j);
+}
+void launch_tv_axpy(const TVTask* t, int n, int base, int blocks, int j, hipStream_t s) {
+  hipLaunchKernelGGL(axpy_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base, j);
+}
+void launch_tv_pupdate(const TVTask* t, int n, int base, int blocks, int j, hipStream_t s) {
+  hipLaunchKernelGGL(pupdate_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base, j);
+}
+void launch_tv_shrink(const TVTask* t, int n, int base, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(shrink_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base);
+}
+void launch_tv_addback(const TVTask* t, int n, int base, int sblocks, hipStream_t s) {
+  hipLaunchKernelGGL(addback_kernel, dim3(sblocks), dim3(NT_), 0, s, t, n, base);
+}
+void launch_tv_init(const TVTask* t, int n, int base, int blocks, int sbase, int sblocks,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(tvinit_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base);
+  hipLaunchKernelGGL(fkinit_kernel, dim3(sblocks), dim3(NT_), 0, s, t, n, sbase);
+}
+
+}  // namespace acu

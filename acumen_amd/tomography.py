@@ -1,0 +1,258 @@
+"""Host-side mirror of Acumen's Fortran module ``Tomography``.
+
+The reference keeps its state in module globals (tomography/tv_hpc.F90:3-7)
+and exposes six subroutines; :class:`Tomography` keeps the same names,
+argument meanings and calling order on top of the C ABI of
+libacumen_hip.so:
+
+    t = Tomography(ctx)                 # the module
+    t.nimage, t.ntheta, t.wpu = N, T, w # globals set by the driver
+    t.int_radon()                       # tv_hpc.F90:11-78
+    t.theta[:] = angles                 # tv_hpc.F90:560-562
+    t.f[:] = sinogram                   # tv_hpc.F90:614-618
+    img_tv, img_fbp = t.tvopt(jslice)   # tv_hpc.F90:162-363
+    img_fbp = t.get_slice()             # fbp_hpc.F90:129-150
+    sino = t.radon(img); img = t.iradon(sino)
+    t.fin_radon()
+
+Arrays are numpy float32 in the reference's Fortran layout expressed as
+C-ordered arrays: an image ``(x, y)`` is shape ``(N, N)`` indexed ``[y, x]``,
+a sinogram ``(detector, angle)`` is shape ``(T, N)`` indexed ``[angle, det]``.
+
+:class:`Batch` is the batched form (many slices of their own N, T, wpu and
+angles) that replaces the reference's MPI master/worker slice loop.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+_F = ctypes.POINTER(ctypes.c_float)
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(_F)
+
+
+def _f32(a, shape=None) -> np.ndarray:
+    out = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and out.shape != tuple(shape):
+        raise ValueError(f"expected shape {tuple(shape)}, got {out.shape}")
+    return out
+
+
+class Context:
+    """One device and the HIP stream all work of this context is ordered on."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self._lib.acu_ctx_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def synchronize(self) -> None:
+        check(self._lib.acu_ctx_synchronize(self.handle))
+
+    def set_timing(self, enable: bool) -> None:
+        check(self._lib.acu_ctx_set_timing(self.handle, 1 if enable else 0))
+
+    def kernel_time(self, kind: int) -> tuple[float, int]:
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        check(self._lib.acu_ctx_kernel_time(self.handle, kind, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.acu_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Slices reconstructed together; a 1-slice batch is a plan."""
+
+    def __init__(self, ctx: Context, slices):
+        """slices: iterable of (nimage, ntheta, wpu, theta_deg or None)."""
+        self.ctx = ctx
+        self._lib = ctx._lib
+        slices = list(slices)
+        if not slices:
+            raise ValueError("empty batch")
+        n = len(slices)
+        Ns = (ctypes.c_int * n)(*[int(s[0]) for s in slices])
+        Ts = (ctypes.c_int * n)(*[int(s[1]) for s in slices])
+        Ws = (ctypes.c_int * n)(*[int(s[2]) for s in slices])
+        self._thetas = [None if s[3] is None else _f32(s[3], (int(s[1]),)) for s in slices]
+        th = (_F * n)(*[(_fptr(t) if t is not None else _F()) for t in self._thetas])
+        h = ctypes.c_void_p()
+        check(self._lib.acu_batch_create(ctx.handle, n, Ns, Ts, Ws, th, ctypes.byref(h)))
+        self.handle = h
+        self.shapes = [(int(s[0]), int(s[1])) for s in slices]
+
+    def __len__(self):
+        return len(self.shapes)
+
+    def set_theta(self, i: int, theta) -> None:
+        t = _f32(theta, (self.shapes[i][1],))
+        check(self._lib.acu_batch_set_theta(self.handle, i, _fptr(t)))
+
+    def set_sino(self, i: int, f) -> None:
+        N, T = self.shapes[i]
+        a = _f32(f, (T, N))
+        check(self._lib.acu_batch_set_sino(self.handle, i, _fptr(a)))
+
+    def get_image(self, i: int, which: int = _lib.ACU_IMG_TV) -> np.ndarray:
+        N, _ = self.shapes[i]
+        out = np.empty((N, N), np.float32)
+        check(self._lib.acu_batch_get_image(self.handle, i, which, _fptr(out)))
+        return out
+
+    def get_delta(self, i: int, n: int) -> np.ndarray:
+        out = np.empty(n, np.float32)
+        check(self._lib.acu_batch_get_delta(self.handle, i, _fptr(out), n))
+        return out
+
+    def device_ptr(self, i: int, which: int) -> int:
+        p = self._lib.acu_batch_device_ptr(self.handle, i, which)
+        if not p:
+            check(_lib.ACU_ERR_ARG)
+        return p
+
+    def fbp(self) -> None:
+        check(self._lib.acu_batch_fbp(self.handle))
+
+    def tvopt(self, mu: float, lam: float, nouter: int, ninner: int = 1, ncg: int = 5,
+              mode: int = _lib.ACU_TV_GRAPH) -> None:
+        check(self._lib.acu_batch_tvopt(self.handle, float(mu), float(lam), int(nouter),
+                                        int(ninner), int(ncg), int(mode)))
+
+    # single-slice operators on host arrays (slice 0) -- the reference subroutines
+    def radon(self, img) -> np.ndarray:
+        N, T = self.shapes[0]
+        x = _f32(img, (N, N))
+        out = np.empty((T, N), np.float32)
+        check(self._lib.acu_radon(self.handle, _fptr(out), _fptr(x)))
+        return out
+
+    def iradon(self, sino) -> np.ndarray:
+        N, T = self.shapes[0]
+        s = _f32(sino, (T, N))
+        out = np.empty((N, N), np.float32)
+        check(self._lib.acu_iradon(self.handle, _fptr(out), _fptr(s)))
+        return out
+
+    def radon_dev(self, i: int, sino_ptr: int, img_ptr: int) -> None:
+        check(self._lib.acu_radon_dev(self.handle, i, sino_ptr, img_ptr))
+
+    def iradon_dev(self, i: int, img_ptr: int, sino_ptr: int) -> None:
+        check(self._lib.acu_iradon_dev(self.handle, i, img_ptr, sino_ptr))
+
+    def ramp_dev(self, i: int, out_ptr: int, sino_ptr: int) -> None:
+        check(self._lib.acu_ramp_dev(self.handle, i, out_ptr, sino_ptr))
+
+    def backproject_dev(self, i: int, img_ptr: int, filtered_ptr: int) -> None:
+        check(self._lib.acu_backproject_dev(self.handle, i, img_ptr, filtered_ptr))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self._lib.acu_batch_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tomography:
+    """The reference module: globals plus int_radon / iradon / radon / tvopt /
+    get_slice / fin_radon (tomography/tv_hpc.F90:1-390)."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or Context()
+        # module globals (tv_hpc.F90:3-4); ninnner keeps the reference's spelling
+        self.ncg = 5
+        self.ninnner = 1
+        self.nouter = 1
+        self.nimage = 0
+        self.ntheta = 0
+        self.wpu = 1
+        self.add_Nx = 0
+        self.mu = 0.0
+        self.lambda_ = 0.0
+        self.theta: np.ndarray | None = None
+        self.f: np.ndarray | None = None
+        self.last_delta: np.ndarray | None = None
+        self._batch: Batch | None = None
+
+    @property
+    def midpoint(self) -> float:
+        return float((self.nimage + 1) // 2)
+
+    def int_radon(self) -> None:
+        """Allocate the device state for (nimage, ntheta, wpu); theta and f are
+        allocated here and filled by the caller afterwards (tv_hpc.F90:559-562)."""
+        if self._batch is not None:
+            self._batch.close()
+        self._batch = Batch(self.ctx, [(self.nimage, self.ntheta, self.wpu, None)])
+        self.theta = np.zeros(self.ntheta, np.float32)
+        self.f = np.zeros((self.ntheta, self.nimage), np.float32)
+
+    def _sync_theta(self) -> None:
+        self._batch.set_theta(0, self.theta)
+
+    def iradon(self, sinogram) -> np.ndarray:
+        """iradon(img_rec, sinogram): ramp filter + masked backprojection."""
+        self._sync_theta()
+        return self._batch.iradon(sinogram)
+
+    def radon(self, img) -> np.ndarray:
+        """radon(sinogram, img): ray-driven bilinear projection / nimage."""
+        self._sync_theta()
+        return self._batch.radon(img)
+
+    def get_slice(self) -> np.ndarray:
+        """get_slice(img_rec_fbp) = iradon(f) (fbp_hpc.F90:129-150)."""
+        self._sync_theta()
+        self._batch.set_sino(0, self.f)
+        self._batch.fbp()
+        return self._batch.get_image(0, _lib.ACU_IMG_FBP)
+
+    def tvopt(self, jslice: int = 0, mode: int = _lib.ACU_TV_GRAPH):
+        """tvopt(img_rec_tv, img_rec_fbp, jslice) with the module's mu,
+        lambda, nouter, ninnner, ncg.  Returns (img_rec_tv, img_rec_fbp)."""
+        self._sync_theta()
+        self._batch.set_sino(0, self.f)
+        self._batch.tvopt(self.mu, self.lambda_, self.nouter, self.ninnner, self.ncg, mode)
+        self.last_delta = self._batch.get_delta(0, self.nouter) if self.nouter > 0 else None
+        return (self._batch.get_image(0, _lib.ACU_IMG_TV),
+                self._batch.get_image(0, _lib.ACU_IMG_FBP))
+
+    def fin_radon(self) -> None:
+        if self._batch is not None:
+            self._batch.close()
+            self._batch = None

@@ -1,0 +1,151 @@
+/*
+ * acumen.h -- C ABI of the MI355X (gfx950) neutron-CT reconstruction library
+ * libacumen_hip.so.
+ *
+ * It replaces the Fortran module `Tomography` of Acumen
+ * (tomography/tv_hpc.F90:1-390, with copies in fbp_hpc.F90:1-154 and
+ * tv_hpc_slice.F90:1-366).  The reference keeps all state in module globals
+ * and OpenACC device arrays; here that state is an opaque handle:
+ *
+ *   reference                               this library
+ *   ------------------------------------    --------------------------------
+ *   int_radon()        tv_hpc.F90:11-78     acu_plan_create / acu_batch_create
+ *   theta(:) fill      tv_hpc.F90:560-562   acu_batch_set_theta
+ *   f(:,:) fill        tv_hpc.F90:614-618   acu_batch_set_sino
+ *   radon(sino, img)   tv_hpc.F90:129-159   acu_radon / acu_radon_dev
+ *   iradon(img, sino)  tv_hpc.F90:84-126    acu_iradon / acu_iradon_dev
+ *   get_slice(img)     fbp_hpc.F90:129-150  acu_get_slice / acu_batch_fbp
+ *   tvopt(tv, fbp, j)  tv_hpc.F90:162-363   acu_tvopt / acu_batch_tvopt
+ *   fin_radon()        tv_hpc.F90:80-82     acu_batch_destroy
+ *
+ * Conventions
+ *  - Arrays are FP32 in the reference's Fortran layout, first index fastest:
+ *    an image img(x, y) is N*N floats with x fastest, a sinogram
+ *    sino(detector, angle) is N*T floats with the detector fastest.
+ *  - N ("nimage") is the odd internal image side = detector bins, T
+ *    ("ntheta") the number of angles, theta in degrees, wpu >= 1 the width of
+ *    the mask's smooth edge in pixels ("Radon_wpu").
+ *  - Every function returns 0 (ACU_OK) or an ACU_ERR_* code; the message of
+ *    the last failure is acu_last_error().  The reference has no error
+ *    reporting at all (tomography/tv_hpc.F90: ierr is never checked).
+ *  - A context owns one device and one HIP stream; every call on a context
+ *    is ordered on that stream.  Functions that take or return host buffers
+ *    synchronise before returning; *_dev functions and acu_batch_* compute
+ *    calls are asynchronous until acu_ctx_synchronize().
+ *  - One context per host thread.
+ */
+#ifndef ACUMEN_H
+#define ACUMEN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACU_OK 0
+#define ACU_ERR_ARG 1     /* invalid argument */
+#define ACU_ERR_HIP 2     /* HIP runtime error */
+#define ACU_ERR_STATE 3   /* call out of order (e.g. image requested before it exists) */
+
+/* image selectors of acu_batch_get_image / acu_batch_device_ptr */
+#define ACU_IMG_TV 0      /* img_rec_tv: result of tvopt (module array uk after tvopt) */
+#define ACU_IMG_FBP 1     /* img_rec_fbp: FBP warm start / get_slice result */
+#define ACU_SINO_F 2      /* module array f: the windowed sinogram input (N*T) */
+
+/* tvopt execution modes */
+#define ACU_TV_PLAIN 0    /* one slice after another, each reduction returned to the host
+                             before the next kernel (the reference's structure, tv_hpc.F90:243-302) */
+#define ACU_TV_BATCHED 1  /* all slices in every launch, scalars stay on the device */
+#define ACU_TV_GRAPH 2    /* ACU_TV_BATCHED with each outer iteration replayed from a HIP graph */
+
+typedef struct acu_ctx acu_ctx;
+typedef struct acu_batch acu_batch;
+
+/* Last error message of the calling thread (never NULL). */
+const char *acu_last_error(void);
+
+/* Context: device selection and the stream all work is ordered on. */
+int acu_ctx_create(int device, acu_ctx **out);
+int acu_ctx_destroy(acu_ctx *ctx);
+int acu_ctx_synchronize(acu_ctx *ctx);
+
+/* int_radon (tomography/tv_hpc.F90:11-78) for a single slice: filter taps,
+ * mask and device arrays.  theta_deg (ntheta floats) may be NULL and be set
+ * later with acu_batch_set_theta, as the reference fills theta after
+ * int_radon (tv_hpc.F90:559-562). */
+int acu_plan_create(acu_ctx *ctx, int nimage, int ntheta, int wpu, const float *theta_deg,
+                    acu_batch **out);
+
+/* int_radon for `nslices` independent slices, each with its own N, T, wpu
+ * and angles -- the unit the batched kernels launch over.  Replaces the
+ * reference's MPI master/worker slice distribution (tv_hpc.F90:583-631). */
+int acu_batch_create(acu_ctx *ctx, int nslices, const int *nimage, const int *ntheta,
+                     const int *wpu, const float *const *theta_deg, acu_batch **out);
+
+/* fin_radon (tv_hpc.F90:80-82): frees every device array of the batch. */
+int acu_batch_destroy(acu_batch *b);
+
+int acu_batch_nslices(const acu_batch *b);
+
+/* theta(1:ntheta) in degrees for one slice (tv_hpc.F90:560-562). */
+int acu_batch_set_theta(acu_batch *b, int slice, const float *theta_deg);
+
+/* Module array f(N,T) of one slice from host memory (tv_hpc.F90:614-618 +
+ * `update device(f)` at :172). */
+int acu_batch_set_sino(acu_batch *b, int slice, const float *f_host);
+
+/* Copy an image (ACU_IMG_TV or ACU_IMG_FBP, N*N floats) of one slice to host
+ * memory (`update host(uk)`, tv_hpc.F90:180,357). */
+int acu_batch_get_image(acu_batch *b, int slice, int which, float *img_host);
+
+/* Device address of a slice's f, FBP or TV array, for callers that keep
+ * their data resident on the GPU.  Returns NULL on error. */
+float *acu_batch_device_ptr(acu_batch *b, int slice, int which);
+
+/* get_slice (fbp_hpc.F90:129-150) for every slice: img_fbp = iradon(f). */
+int acu_batch_fbp(acu_batch *b);
+
+/* tvopt (tv_hpc.F90:162-363) for every slice: FBP warm start into
+ * ACU_IMG_FBP, split-Bregman TV into ACU_IMG_TV, with the module parameters
+ * mu, lambda, nouter, ninnner (sic) and ncg of the reference.  mode is one of
+ * ACU_TV_*.  ncg <= 60. */
+int acu_batch_tvopt(acu_batch *b, float mu, float lambda, int nouter, int ninner, int ncg,
+                    int mode);
+
+/* Err_diff (the reference's "delta", tv_hpc.F90:327-333,350-352) after each
+ * outer iteration of the last tvopt, n <= nouter values. */
+int acu_batch_get_delta(acu_batch *b, int slice, float *delta_host, int n);
+
+/* radon(sinogram, img) (tv_hpc.F90:129-159) on device memory of slice
+ * `slice`: img_dev N*N in, sino_dev N*T out.  Asynchronous. */
+int acu_radon_dev(acu_batch *b, int slice, float *sino_dev, const float *img_dev);
+
+/* iradon(img_rec, sinogram) (tv_hpc.F90:84-126) on device memory: ramp
+ * filter then masked backprojection.  sino_dev N*T in, img_dev N*N out. */
+int acu_iradon_dev(acu_batch *b, int slice, float *img_dev, const float *sino_dev);
+
+/* The two halves of iradon: ramp filter (tv_hpc.F90:93-105) and masked
+ * backprojection of an already filtered sinogram (:107-123). */
+int acu_ramp_dev(acu_batch *b, int slice, float *out_dev, const float *sino_dev);
+int acu_backproject_dev(acu_batch *b, int slice, float *img_dev, const float *filtered_dev);
+
+/* Host-memory forms of the reference subroutines on a single-slice plan
+ * (slice 0).  Synchronous. */
+int acu_radon(acu_batch *b, float *sino_host, const float *img_host);
+int acu_iradon(acu_batch *b, float *img_host, const float *sino_host);
+int acu_get_slice(acu_batch *b, const float *f_host, float *img_fbp_host);
+int acu_tvopt(acu_batch *b, const float *f_host, float mu, float lambda, int nouter, int ninner,
+              int ncg, float *img_tv_host, float *img_fbp_host, float *delta_host);
+
+/* Kernel timing: when enabled, every ramp-filter, backprojection, cells and
+ * forward-projection launch is bracketed by HIP events on the context's
+ * stream (graph mode is then run as ACU_TV_BATCHED).  acu_ctx_kernel_time
+ * reports the accumulated milliseconds and launch count of one kind:
+ * 0 = ramp filter, 1 = backprojection, 2 = cells pass, 3 = forward projection. */
+int acu_ctx_set_timing(acu_ctx *ctx, int enable);
+int acu_ctx_kernel_time(acu_ctx *ctx, int kind, double *total_ms, long long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACUMEN_H */

@@ -1,0 +1,342 @@
+/*
+ * oracle_body.h -- body of the CPU restatement of Acumen's `Tomography`
+ * module, included twice by acumen_oracle.c: once with REAL=float (the
+ * reference's own FP32 arithmetic, operation by operation, no FMA
+ * contraction) and once with REAL=double (the FP64 truth).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links this.
+ *
+ * Array conventions follow the reference: Fortran column-major, first index
+ * fastest, 1-based indices in the formulas below and 0-based in memory:
+ *   image    (x, y)      -> img [(x-1) + (y-1)*N]
+ *   sinogram (det, angle)-> sino[(d-1) + (a-1)*N]
+ *
+ * Every function cites the reference lines it restates
+ * (tomography/tv_hpc.F90 unless stated otherwise).
+ */
+
+#ifndef REAL
+#error "include through acumen_oracle.c"
+#endif
+
+#define RC(x) ((REAL)(x))
+
+#ifndef ORC_TV_ONLY
+
+/* pi = 4.0*atan(1.0)  (tv_hpc.F90:16) */
+static REAL FN(pi)(void) { return RC(4.0) * ATAN(RC(1.0)); }
+
+/* midpoint = (nimage+1)/2 with integer division (tv_hpc.F90:15) */
+static REAL FN(midpoint)(int N) { return (REAL)((N + 1) / 2); }
+
+/* Ramp filter taps, tv_hpc.F90:53-56:
+ *   filter(jx) = -2.0*N / (pi**2 * (4*(jx-1-N)**2 - 1)),  jx = 1..2N+1
+ * The integer expression 4k^2-1 is evaluated in integer arithmetic. */
+void FN(filter)(int N, REAL *h)
+{
+    const REAL pi = FN(pi)();
+    const int nfilt = 2 * N + 1;
+    for (int jx = 1; jx <= nfilt; ++jx) {
+        const int k = jx - 1 - N;
+        const int q = 4 * k * k - 1;
+        h[jx - 1] = (-(RC(2.0) * (REAL)N)) / ((pi * pi) * (REAL)q);
+    }
+}
+
+/* Circular smooth mask, tv_hpc.F90:58-74.  wpu must be >= 1. */
+void FN(cmask)(int N, int T, int wpu, REAL *cmask)
+{
+    const REAL pi = FN(pi)();
+    const REAL rad_n = (REAL)(N - 1) / RC(2.0) - (REAL)wpu;
+    const REAL half = ((REAL)N - RC(1.0)) / RC(2.0);
+    for (int jx = 1; jx <= N; ++jx) {
+        for (int jy = 1; jy <= N; ++jy) {
+            const REAL dx = ((REAL)jx - half) - RC(1.0);
+            const REAL dy = ((REAL)jy - half) - RC(1.0);
+            const REAL rr = SQRT(dx * dx + dy * dy);
+            const REAL rtmp = RC(1.0) - (RC(2.0) * (rr - rad_n)) / (REAL)wpu;
+            REAL v;
+            if (rtmp <= RC(0.0)) {
+                v = RC(0.0);
+            } else if (rtmp >= RC(1.0)) {
+                v = pi / (REAL)T;
+            } else {
+                const REAL e1 = EXP(RC(1.0) - RC(1.0) / (RC(1.0) - rtmp));
+                v = (pi * EXP(RC(1.0) - RC(1.0) / (RC(1.0) - e1))) / (REAL)T;
+            }
+            cmask[(jx - 1) + (size_t)(jy - 1) * N] = v;
+        }
+    }
+}
+
+/* Ramp filtering (first half of iradon), tv_hpc.F90:93-105:
+ *   sinoCS(jy,t) = sum_{jx=1..N} sino(jx,t) * filter(jy-jx+1+N)
+ * summed sequentially in jx.  The guard jk in (1, 2N+2) is kept. */
+void FN(ramp)(int N, int T, const REAL *h, const REAL *sino, REAL *sinoCS)
+{
+    const int nfilt = 2 * N + 1;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+    for (int jt = 1; jt <= T; ++jt) {
+        const REAL *s = sino + (size_t)(jt - 1) * N;
+        for (int jy = 1; jy <= N; ++jy) {
+            REAL sum = RC(0.0);
+            for (int jx = 1; jx <= N; ++jx) {
+                const int jk = jy - jx + 1 + N;
+                if (jk > 1 && jk < nfilt + 1)
+                    sum = sum + s[jx - 1] * h[jk - 1];
+            }
+            sinoCS[(jy - 1) + (size_t)(jt - 1) * N] = sum;
+        }
+    }
+}
+
+/* Per-angle sin/cos of ctheta = pi*(90-theta)/180 (backprojection,
+ * tv_hpc.F90:113) or pi*(theta-90)/180 (projection, tv_hpc.F90:142).
+ * sin and cos are pure functions of ctheta, so hoisting them out of the
+ * pixel loops leaves every value unchanged. */
+static void FN(angles)(int T, const float *theta, int bp, REAL *sn, REAL *cs)
+{
+    const REAL pi = FN(pi)();
+    for (int j = 0; j < T; ++j) {
+        const REAL th = (REAL)theta[j];
+        const REAL ct = bp ? (pi * (RC(90.0) - th)) / RC(180.0)
+                           : (pi * (th - RC(90.0))) / RC(180.0);
+        sn[j] = SIN(ct);
+        cs[j] = COS(ct);
+        if (g_fp32_angles) {
+            /* the reference's FP32 sin/cos of its FP32 ctheta, promoted */
+            const float pf = 4.0f * atanf(1.0f), tf = theta[j];
+            const float cf = bp ? (pf * (90.0f - tf)) / 180.0f : (pf * (tf - 90.0f)) / 180.0f;
+            sn[j] = (REAL)sinf(cf);
+            cs[j] = (REAL)cosf(cf);
+        }
+    }
+}
+
+/* Pixel-driven linear-interpolation backprojection (second half of
+ * iradon), tv_hpc.F90:107-123:
+ *   yr = sin(c)*(x-m) + cos(c)*(y-m) + m,  y1 = floor(yr), y2 = y1+1,
+ *   taps used iff y1 > 1 and y2 < N+1,
+ *   img(x,y) = cmask(x,y) * sum_t [S(y1)*(y2-yr) + S(y2)*(yr-y1)]. */
+void FN(backproject)(int N, int T, const float *theta, const REAL *cmask,
+                     const REAL *sinoCS, REAL *img)
+{
+    REAL *sn = (REAL *)malloc(sizeof(REAL) * (size_t)T);
+    REAL *cs = (REAL *)malloc(sizeof(REAL) * (size_t)T);
+    FN(angles)(T, theta, 1, sn, cs);
+    const REAL m = FN(midpoint)(N);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+    for (int jy = 1; jy <= N; ++jy) {
+        for (int jx = 1; jx <= N; ++jx) {
+            REAL sum = RC(0.0);
+            for (int jt = 1; jt <= T; ++jt) {
+                const REAL yr = (sn[jt - 1] * ((REAL)jx - m) + cs[jt - 1] * ((REAL)jy - m)) + m;
+                const int y1 = (int)FLOOR(yr);
+                const int y2 = y1 + 1;
+                if (y1 > 1 && y2 < N + 1) {
+                    const REAL *S = sinoCS + (size_t)(jt - 1) * N;
+                    sum = sum + S[y1 - 1] * ((REAL)y2 - yr);
+                    sum = sum + S[y2 - 1] * (yr - (REAL)y1);
+                }
+            }
+            const size_t o = (size_t)(jx - 1) + (size_t)(jy - 1) * N;
+            img[o] = cmask[o] * sum;
+        }
+    }
+    free(sn);
+    free(cs);
+}
+
+/* iradon = ramp filter + backprojection (tv_hpc.F90:84-126). */
+void FN(iradon)(int N, int T, const float *theta, const REAL *h, const REAL *cmask,
+                const REAL *sino, REAL *img)
+{
+    REAL *cs = (REAL *)malloc(sizeof(REAL) * (size_t)N * T);
+    FN(ramp)(N, T, h, sino, cs);
+    FN(backproject)(N, T, theta, cmask, cs, img);
+    free(cs);
+}
+
+/* Ray-driven bilinear forward projection, tv_hpc.F90:129-159:
+ *   xr = cos(c)*(x-m) - sin(c)*(y-m) + m,  yr = sin(c)*(x-m) + cos(c)*(y-m) + m
+ *   taps used iff min(x1,y1) > 1 and max(x2,y2) < N+1,
+ *   sino(y,t) = (1/N) * sum_{x=1..N} bilinear(img; xr, yr), x in order. */
+void FN(radon)(int N, int T, const float *theta, const REAL *img, REAL *sino)
+{
+    REAL *sn = (REAL *)malloc(sizeof(REAL) * (size_t)T);
+    REAL *cs = (REAL *)malloc(sizeof(REAL) * (size_t)T);
+    FN(angles)(T, theta, 0, sn, cs);
+    const REAL m = FN(midpoint)(N);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+    for (int jt = 1; jt <= T; ++jt) {
+        const REAL s = sn[jt - 1], c = cs[jt - 1];
+        for (int jy = 1; jy <= N; ++jy) {
+            REAL sum = RC(0.0);
+            for (int jx = 1; jx <= N; ++jx) {
+                const REAL xr = (c * ((REAL)jx - m) - s * ((REAL)jy - m)) + m;
+                const REAL yr = (s * ((REAL)jx - m) + c * ((REAL)jy - m)) + m;
+                const int y1 = (int)FLOOR(yr), y2 = y1 + 1;
+                const int x1 = (int)FLOOR(xr), x2 = x1 + 1;
+                const int mn = x1 < y1 ? x1 : y1;
+                const int mx = x2 > y2 ? x2 : y2;
+                if (mn > 1 && mx < N + 1) {
+                    const REAL i11 = img[(x1 - 1) + (size_t)(y1 - 1) * N];
+                    const REAL i21 = img[(x2 - 1) + (size_t)(y1 - 1) * N];
+                    const REAL i12 = img[(x1 - 1) + (size_t)(y2 - 1) * N];
+                    const REAL i22 = img[(x2 - 1) + (size_t)(y2 - 1) * N];
+                    const REAL ax = (REAL)x2 - xr, bx = xr - (REAL)x1;
+                    const REAL ay = (REAL)y2 - yr, by = yr - (REAL)y1;
+                    sum = sum + (i11 * ax) * ay;
+                    sum = sum + (i21 * bx) * ay;
+                    sum = sum + (i12 * ax) * by;
+                    sum = sum + (i22 * bx) * by;
+                }
+            }
+            sino[(jy - 1) + (size_t)(jt - 1) * N] = sum / (REAL)N;
+        }
+    }
+    free(sn);
+    free(cs);
+}
+
+#endif /* ORC_TV_ONLY */
+
+/* Periodic neighbour indices of tvopt (tv_hpc.F90:220,234-237), derived from
+ * FP32 floor formulas that reduce exactly to wrap-around for N < 2^24. */
+static inline int TVFN(nxt)(int j, int N) { return j == N ? 1 : j + 1; }
+static inline int TVFN(prv)(int j, int N) { return j == 1 ? N : j - 1; }
+
+/* Split-Bregman TV with inner CG, tv_hpc.F90:162-363.
+ *   f         in  (N,T) windowed sinogram (module array f)
+ *   img_tv    out (N,N) TV result; img_fbp out (N,N) FBP warm start
+ *   delta_log out (nouter) optional Err_diff after each outer iteration
+ * The scalar reductions (rsold, num, rsnew, Err_diff) are sequential sums in
+ * the array's memory order, in REAL, as in the reference. */
+void TVFN(tvopt)(int N, int T, const float *theta, int wpu, const float *f_in,
+               REAL mu, REAL lambda, int nouter, int ninner, int ncg,
+               REAL *img_tv, REAL *img_fbp, REAL *delta_log)
+{
+    const size_t NN = (size_t)N * N, NT = (size_t)N * T;
+    REAL *h = (REAL *)malloc(sizeof(REAL) * (size_t)(2 * N + 1));
+    REAL *cmask = (REAL *)malloc(sizeof(REAL) * NN);
+    FN(filter)(N, h);
+    FN(cmask)(N, T, wpu, cmask);
+    REAL *uk = (REAL *)calloc(NN, sizeof(REAL)), *ukp = (REAL *)calloc(NN, sizeof(REAL));
+    REAL *dkx = (REAL *)calloc(NN, sizeof(REAL)), *dky = (REAL *)calloc(NN, sizeof(REAL));
+    REAL *bkx = (REAL *)calloc(NN, sizeof(REAL)), *bky = (REAL *)calloc(NN, sizeof(REAL));
+    REAL *ifkt = (REAL *)calloc(NN, sizeof(REAL)), *rhs = (REAL *)calloc(NN, sizeof(REAL));
+    REAL *iukt = (REAL *)calloc(NN, sizeof(REAL)), *r = (REAL *)calloc(NN, sizeof(REAL));
+    REAL *p = (REAL *)calloc(NN, sizeof(REAL)), *irpt = (REAL *)calloc(NN, sizeof(REAL));
+    REAL *Ap = (REAL *)calloc(NN, sizeof(REAL));
+    REAL *f = (REAL *)malloc(sizeof(REAL) * NT), *fk = (REAL *)malloc(sizeof(REAL) * NT);
+    REAL *ruk = (REAL *)malloc(sizeof(REAL) * NT), *rp = (REAL *)malloc(sizeof(REAL) * NT);
+    for (size_t i = 0; i < NT; ++i) f[i] = (REAL)f_in[i];
+
+#define IX(x, y) ((size_t)((x) - 1) + (size_t)((y) - 1) * N)
+    /* FBP warm start, :176-184 */
+    FN(iradon)(N, T, theta, h, cmask, f, uk);
+    for (size_t i = 0; i < NN; ++i) img_fbp[i] = uk[i];
+    /* :187-202 */
+    for (size_t i = 0; i < NN; ++i) dkx[i] = dky[i] = bkx[i] = bky[i] = RC(0.0);
+    for (size_t i = 0; i < NT; ++i) fk[i] = f[i];
+
+    const REAL four = RC(4.0);
+    const REAL inv_lambda = RC(1.0) / lambda;
+    DREAL err_diff = (DREAL)0;
+    for (int jouter = 1; jouter <= nouter; ++jouter) {
+        for (int jinner = 1; jinner <= ninner; ++jinner) {
+            for (size_t i = 0; i < NN; ++i) ukp[i] = uk[i];           /* :207-212 */
+            FN(iradon)(N, T, theta, h, cmask, fk, ifkt);               /* :215 */
+            for (int y = 1; y <= N; ++y)                               /* :217-225 */
+                for (int x = 1; x <= N; ++x) {
+                    const int xl = TVFN(nxt)(x, N), yl = TVFN(nxt)(y, N);
+                    REAL t = dkx[IX(xl, y)] - bkx[IX(xl, y)];
+                    t = t - dkx[IX(x, y)];
+                    t = t + bkx[IX(x, y)];
+                    t = t + dky[IX(x, yl)];
+                    t = t - bky[IX(x, yl)];
+                    t = t - dky[IX(x, y)];
+                    t = t + bky[IX(x, y)];
+                    rhs[IX(x, y)] = mu * ifkt[IX(x, y)] + lambda * t;
+                }
+            FN(radon)(N, T, theta, uk, ruk);                           /* :228 */
+            FN(iradon)(N, T, theta, h, cmask, ruk, iukt);              /* :229 */
+            for (int y = 1; y <= N; ++y)                               /* :231-241 */
+                for (int x = 1; x <= N; ++x) {
+                    const int xr = TVFN(nxt)(x, N), yr = TVFN(nxt)(y, N);
+                    const int xl = TVFN(prv)(x, N), yl = TVFN(prv)(y, N);
+                    REAL lap = four * uk[IX(x, y)] - uk[IX(xl, y)];
+                    lap = lap - uk[IX(xr, y)];
+                    lap = lap - uk[IX(x, yl)];
+                    lap = lap - uk[IX(x, yr)];
+                    const REAL v = (rhs[IX(x, y)] - mu * iukt[IX(x, y)]) - lambda * lap;
+                    r[IX(x, y)] = v;
+                    p[IX(x, y)] = v;
+                }
+            DREAL rsold = (DREAL)0;                                    /* :243-249 */
+            for (size_t i = 0; i < NN; ++i) rsold = rsold + (DREAL)r[i] * (DREAL)r[i];
+
+            for (int jncg = 1; jncg <= ncg; ++jncg) {                  /* :252-303 */
+                FN(radon)(N, T, theta, p, rp);
+                FN(iradon)(N, T, theta, h, cmask, rp, irpt);
+                for (int y = 1; y <= N; ++y)
+                    for (int x = 1; x <= N; ++x) {
+                        const int xr = TVFN(nxt)(x, N), yr = TVFN(nxt)(y, N);
+                        const int xl = TVFN(prv)(x, N), yl = TVFN(prv)(y, N);
+                        REAL lap = four * p[IX(x, y)] - p[IX(xl, y)];
+                        lap = lap - p[IX(xr, y)];
+                        lap = lap - p[IX(x, yl)];
+                        lap = lap - p[IX(x, yr)];
+                        Ap[IX(x, y)] = mu * irpt[IX(x, y)] + lambda * lap;
+                    }
+                DREAL num = (DREAL)0;
+                for (size_t i = 0; i < NN; ++i) num = num + (DREAL)p[i] * (DREAL)Ap[i];
+                const REAL alpha = (REAL)(rsold / num);
+                for (size_t i = 0; i < NN; ++i) {
+                    uk[i] = uk[i] + alpha * p[i];
+                    r[i] = r[i] - alpha * Ap[i];
+                }
+                DREAL rsnew = (DREAL)0;
+                for (size_t i = 0; i < NN; ++i) rsnew = rsnew + (DREAL)r[i] * (DREAL)r[i];
+                const REAL beta = (REAL)(rsnew / rsold);
+                for (size_t i = 0; i < NN; ++i) p[i] = r[i] + beta * p[i];
+                rsold = rsnew;
+            }
+            for (int y = 1; y <= N; ++y)                               /* :305-325 */
+                for (int x = 1; x <= N; ++x) {
+                    const int xl = TVFN(prv)(x, N), yl = TVFN(prv)(y, N);
+                    REAL t = (uk[IX(xl, y)] - uk[IX(x, y)]) + bkx[IX(x, y)];
+                    REAL d;
+                    if (t == RC(0.0)) d = RC(0.0);
+                    else {
+                        REAL a = FABS(t) - inv_lambda;
+                        if (!(a > RC(0.0))) a = RC(0.0);
+                        d = (a * t) / FABS(t);
+                    }
+                    dkx[IX(x, y)] = d;
+                    bkx[IX(x, y)] = t - d;
+                    t = (bky[IX(x, y)] + uk[IX(x, yl)]) - uk[IX(x, y)];
+                    if (t == RC(0.0)) d = RC(0.0);
+                    else {
+                        REAL a = FABS(t) - inv_lambda;
+                        if (!(a > RC(0.0))) a = RC(0.0);
+                        d = (a * t) / FABS(t);
+                    }
+                    dky[IX(x, y)] = d;
+                    bky[IX(x, y)] = t - d;
+                }
+            err_diff = (DREAL)0;                                       /* :327-333 */
+            for (size_t i = 0; i < NN; ++i) err_diff = err_diff + (DREAL)FABS(uk[i] - ukp[i]);
+        }
+        FN(radon)(N, T, theta, uk, ruk);                               /* :339 */
+        for (size_t i = 0; i < NT; ++i) fk[i] = (fk[i] + f[i]) - ruk[i]; /* :341-346 */
+        if (delta_log) delta_log[jouter - 1] = (REAL)err_diff;
+    }
+#undef IX
+    for (size_t i = 0; i < NN; ++i) img_tv[i] = uk[i];
+    free(h); free(cmask); free(uk); free(ukp); free(dkx); free(dky); free(bkx); free(bky);
+    free(ifkt); free(rhs); free(iukt); free(r); free(p); free(irpt); free(Ap);
+    free(f); free(fk); free(ruk); free(rp);
+}
+
+#undef RC
