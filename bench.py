@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Headline benchmark: split-Bregman TV slices/s at 2048^2 on one MI355X.
+
+Workload (one slice): 2048 detector bins (odd-padded to N = 2049) x 1500
+angles on [0, 180), a 2048^2 phantom of 40 seeded ellipses, Poisson noise at
+I0 = 5e3, Ram-Lak FBP warm start then split-Bregman TV with mu = 0.05,
+lambda = 0.1, 50 outer x 1 inner x 5 CG iterations -- FP32 operators, FP64
+CG scalars.  A step is that reconstruction for a batch of `--batch` slices
+resident in HBM, run by the batched path (every kernel launch covers all
+slices, CG scalars stay on the device).  `value` is slices/s over the timed
+steps.  Alongside it: the plain per-slice GPU path (one slice at a time, each
+reduction returned to the host before the next kernel, as the reference is
+structured) and FBP slices/s of the batched path.
+
+roofline: the forward projector dominates TV time (6 of the 13 projector
+launches per outer iteration and ~85% of the time); its algorithmic work is
+14 FP32 flops per ray sample (2 coordinates, 2 fractions, 3 lerps, 1
+accumulate) over N*N*T samples per slice, measured with HIP events on the
+library's stream during the timed steps.
+
+cpu_baseline: the oracle's FP32 restatement of the reference's loops, one
+slice per host thread as the reference's MPI ranks do, on a bounded sample
+(one radon + one iradon per thread at 1/10 of the angles), extrapolated to a
+TV slice (350 radon + 351 iradon).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+N_DET = 2048
+T_ANG = 1500
+MU, LAM, NOUTER, NINNER, NCG = 0.05, 0.1, 50, 1, 5
+FP_FLOPS_PER_SAMPLE = 14
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_inputs(ctx, batch, N, T, seed0):
+    """Phantoms -> GPU radon -> Poisson noise (host), per slice."""
+    import numpy as np
+
+    from acumen_amd import Batch, synth
+    th = synth.angles(T)
+    wpu = synth.default_wpu(N)
+    sinos = []
+    with Batch(ctx, [(N, T, wpu, th)]) as p:
+        for i in range(batch):
+            x = synth.phantom(N, 40, seed0 + i)
+            s = p.radon(x)
+            sinos.append(synth.poisson_noise(s, 5e3, N, seed0 + 100 + i))
+    return th, wpu, sinos
+
+
+def traffic_from_profile():
+    """HBM bytes per forward-projection launch from the committed rocprofv3
+    counter summary, if one is present."""
+    path = os.path.join(ROOT, "profiles", "fp_kernel_traffic_2049x1500.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch_per_slice:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(N, T, wpu, threads):
+    """Oracle FP32 port: one slice per thread, one radon + one iradon at T/10
+    angles, extrapolated to a full TV slice.  Returns (slices/s, sample)."""
+    import numpy as np
+
+    from acumen_amd import synth
+    from oracle import ref
+    Ts = T // 10
+    th = synth.angles(Ts)
+    imgs = np.stack([synth.phantom(N, 40, 7000 + i) for i in range(threads)])
+    ref.build()
+    t0 = time.time()
+    ref.pass_batch_f32(imgs, th, wpu, threads)
+    wall = time.time() - t0
+    per_pass = wall * (T / Ts)                 # one radon + one iradon at T angles
+    per_slice = per_pass * (7 * NOUTER + 0.5)  # 350 radon + 351 iradon
+    sample = (f"{threads} slices in parallel, one per thread, each one radon + one iradon "
+              f"at {N}x{N} x {Ts} angles (1/10 of {T}): {wall:.1f} s wall; extrapolated "
+              f"x{T // Ts} in angles and x{7 * NOUTER + 0.5} passes per TV slice")
+    return threads / per_slice, sample, wall
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=4, help="slices per step")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-slice leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(slices are independent; multi-GPU sharding is described in DESIGN.md)",
+              file=sys.stderr)
+        return 2
+    if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
+        print("bench.py: multi-process launch is not supported (one GPU)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    from acumen_amd import ACU_IMG_TV, ACU_TV_BATCHED, ACU_TV_PLAIN, Batch, Context, _lib, synth
+
+    N = synth.padded_size(N_DET)
+    T = T_ANG
+    B = args.batch
+    ctx = Context(0)
+    log(f"inputs: {B} slices of {N}x{N}, {T} angles")
+    th, wpu, sinos = make_inputs(ctx, B, N, T, 1001)
+    batch = Batch(ctx, [(N, T, wpu, th)] * B)
+    for i, f in enumerate(sinos):
+        batch.set_sino(i, f)
+    ctx.synchronize()
+
+    for w in range(args.warmup):
+        log(f"warmup step {w + 1}/{args.warmup}")
+        batch.tvopt(MU, LAM, NOUTER, NINNER, NCG, ACU_TV_BATCHED)
+        ctx.synchronize()
+
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        batch.tvopt(MU, LAM, NOUTER, NINNER, NCG, ACU_TV_BATCHED)
+        ctx.synchronize()
+        log(f"step {k + 1}/{args.steps} done at {time.perf_counter() - t0:.2f} s")
+    elapsed = time.perf_counter() - t0
+    kt = {name: ctx.kernel_time(kind) for name, kind in
+          (("ramp", _lib.KIND_RAMP), ("bp", _lib.KIND_BP), ("cells", _lib.KIND_CELLS),
+           ("fp", _lib.KIND_FP))}
+    ctx.set_timing(False)
+    tv_img = batch.get_image(0, ACU_IMG_TV)
+    assert np.all(np.isfinite(tv_img)), "non-finite TV result"
+
+    slices = B * args.steps
+    value = slices / elapsed
+    ms_per_step = elapsed / args.steps * 1e3
+
+    # FBP slices/s of the batched path
+    reps = 5
+    batch.fbp()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        batch.fbp()
+    ctx.synchronize()
+    fbp_rate = B * reps / (time.perf_counter() - t1)
+
+    plain_rate = None
+    if not args.no_plain:
+        log("plain per-slice path: 1 slice")
+        with Batch(ctx, [(N, T, wpu, th)]) as one:
+            one.set_sino(0, sinos[0])
+            ctx.synchronize()
+            t2 = time.perf_counter()
+            one.tvopt(MU, LAM, NOUTER, NINNER, NCG, ACU_TV_PLAIN)
+            ctx.synchronize()
+            plain_rate = 1.0 / (time.perf_counter() - t2)
+    batch.close()
+    ctx.close()
+
+    fp_ms, fp_n = kt["fp"]
+    fp_avg_ms = fp_ms / max(fp_n, 1)
+    samples_per_launch = float(N) * N * T * B
+    achieved = FP_FLOPS_PER_SAMPLE * samples_per_launch / (fp_avg_ms * 1e-3) / 1e12
+    traffic = traffic_from_profile()
+    roofline = {
+        "bound": "mfma",
+        "kernel": "fp_kernel (ray-driven bilinear projection on the FP32 VALU; compute roofline)",
+        "achieved": round(achieved, 3),
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+        "traffic": None if traffic is None else traffic * B,
+        "avg_launch_ms": round(fp_avg_ms, 4),
+        "launches": fp_n,
+        "work_per_launch": f"{FP_FLOPS_PER_SAMPLE} flop x {N}*{N}*{T} samples x {B} slices",
+    }
+
+    cpu = None
+    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        log(f"cpu baseline: {args.cpu_threads} threads")
+        rate, sample, wall = cpu_baseline(N, T, wpu, args.cpu_threads)
+        cpu = {"value": rate, "unit": "slices/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": sample}
+
+    out = {
+        "metric": "TV-SB slices/s batched vs plain @2048^2 (rel-L2<=1e-4 vs FP64); FBP slices/s",
+        "value": round(value, 5),
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"split-Bregman TV, {B} slices of {N_DET}^2 (N={N}) x {T} angles, "
+                               f"I0=5e3, mu={MU}, lambda={LAM}, {NOUTER} outer x {NCG} CG, "
+                               "FBP warm start, batched path",
+                   "slices_per_step": B, "nimage": N, "ntheta": T, "nouter": NOUTER,
+                   "ncg": NCG, "wpu": wpu},
+        "plain_slices_per_s": None if plain_rate is None else round(plain_rate, 5),
+        "batched_vs_plain": None if plain_rate is None else round(value / plain_rate, 3),
+        "fbp_slices_per_s": round(fbp_rate, 3),
+        "kernel_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
