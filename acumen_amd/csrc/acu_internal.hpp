@@ -65,6 +65,7 @@ constexpr int BP_W = 76;         // detector bins staged per angle (>= ceil(hypo
 
 // ---- forward projection (fp.hip) ----
 constexpr int FP_THREADS = 256;  // rays per workgroup (one angle per workgroup)
+constexpr int FP_CHUNK_DEFAULT = 1 << 20;  // steps per sweep chunk (default: the whole ray)
 
 // ---- ramp filter GEMM (filter.hip) ----
 constexpr int RF_BM = 64;   // detector rows per tile
@@ -93,7 +94,11 @@ struct FPTask {
   int N, T;
   float mid;
   int block_begin;       // first global block index of this slice
-  int nrb;               // ray blocks per angle = ceil(N / FP_THREADS); block = rb * T + angle
+  int nrb;               // ray blocks per angle = ceil(N / FP_THREADS)
+  int nchunk;            // sweep chunks per ray = ceil(N / clen); block = (chunk*nrb + rb)*T + angle
+  int clen;              // steps per chunk
+  int sum_begin;         // first global block index of this slice in the chunk-sum pass
+  float* partial;        // nchunk*T*N per-chunk partial ray sums
   const float4* cells;   // (N+1)^2 cells, x-major
   const float4* cellsT;  // (N+1)^2 cells, y-major (transposed)
   const float2* sc;      // per-angle (sin, cos) for projection
@@ -123,7 +128,8 @@ struct CellTask {
 void launch_ramp(const RFTask* d_tasks, int ntasks, int base, int tiles, hipStream_t s);
 void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, hipStream_t s);
 void launch_cells(const CellTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
-void launch_forward(const FPTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
+void launch_forward(const FPTask* d_tasks, int ntasks, int base, int blocks, int sum_base,
+                    int sum_blocks, hipStream_t s);
 
 // Host-side helpers.
 int bp_tiles_x(int N);
@@ -131,6 +137,8 @@ int bp_tiles_y(int N);
 int rf_tiles_m(int N);
 int rf_tiles_n(int T);
 int fp_ray_blocks(int N);
+int fp_chunks(int N, int clen);
+int fp_sum_blocks(int N, int T);
 int cell_blocks(int N);
 
 }  // namespace acu

@@ -17,7 +17,6 @@ struct TVTask {
   float *f, *fk, *ruk;
   double* scal;          // 128 doubles: rs[0..ncg], pAp at 64.., Err_diff at 127
   double* partials;      // nb doubles
-  unsigned* counter;     // reduction completion counter (left at 0)
   float* delta_log;      // delta_cap floats or null
   int delta_cap;
 };
@@ -27,6 +26,7 @@ constexpr int SCAL_STRIDE = 128;
 int ew_blocks(size_t n);
 // `base` is the first global block index of the launch: 0 for a launch over
 // the whole batch, the slice's own begin index for a single-slice launch.
+void launch_tv_finalize(const TVTask* t, int first, int count, int slot, hipStream_t s);
 void launch_tv_residual(const TVTask* t, int n, int base, int blocks, hipStream_t s);
 void launch_tv_apdot(const TVTask* t, int n, int base, int blocks, int j, hipStream_t s);
 void launch_tv_axpy(const TVTask* t, int n, int base, int blocks, int j, hipStream_t s);

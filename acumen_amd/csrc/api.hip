@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -144,18 +145,19 @@ struct Slice {
   uint8_t* tact = nullptr;
   float *f = nullptr, *sinoCS = nullptr, *fbp = nullptr, *ruk = nullptr, *rp = nullptr;
   float4 *cells = nullptr, *cellsT = nullptr;
+  float* fpart = nullptr;  // per-chunk partial ray sums of the forward projector
+  int fp_clen = FP_CHUNK_DEFAULT;
   // TV arena
   float *uk = nullptr, *ukp = nullptr, *dkx = nullptr, *dky = nullptr, *bkx = nullptr,
         *bky = nullptr, *ifkt = nullptr, *iukt = nullptr, *r = nullptr, *p = nullptr,
         *irpt = nullptr, *Ap = nullptr, *fk = nullptr;
   double *scal = nullptr, *partials = nullptr;
-  unsigned* counter = nullptr;
   float* delta = nullptr;
   int delta_cap = 0;
   bool fbp_valid = false, tv_valid = false;
   // global block ranges in the batched launches
   int rf_begin = 0, rf_n = 0, bp_begin = 0, bp_n = 0, cl_begin = 0, cl_n = 0,
-      fp_begin = 0, fp_n = 0, ew_begin = 0, ew_n = 0, sw_begin = 0, sw_n = 0;
+      fp_begin = 0, fp_n = 0, fs_begin = 0, fs_n = 0, ew_begin = 0, ew_n = 0, sw_begin = 0, sw_n = 0;
 };
 
 }  // namespace
@@ -167,7 +169,8 @@ struct acu_batch {
   void* ops_mem = nullptr;
   void* tv_mem = nullptr;
   int tv_delta_cap = 0;
-  int rf_total = 0, bp_total = 0, cl_total = 0, fp_total = 0, ew_total = 0, sw_total = 0;
+  int rf_total = 0, bp_total = 0, cl_total = 0, fp_total = 0, fs_total = 0, ew_total = 0,
+      sw_total = 0;
   // task sets for the fixed operand combinations of FBP / TV
   acu::DevArray<acu::RFTask> rf_f, rf_fk, rf_ruk, rf_rp;
   acu::DevArray<acu::BPTask> bp_fbp, bp_ifkt, bp_iukt, bp_irpt;
@@ -225,6 +228,7 @@ void layout_ops(acu_batch* b, Carver& cv) {
     s.rp = cv.take<float>(NT);
     s.cells = cv.take<float4>(P * P);
     s.cellsT = cv.take<float4>(P * P);
+    s.fpart = cv.take<float>((size_t)fp_chunks(s.N, s.fp_clen) * NT);
   }
 }
 
@@ -237,7 +241,6 @@ void layout_tv(acu_batch* b, Carver& cv, int delta_cap) {
     s.fk = cv.take<float>(NT);
     s.scal = cv.take<double>(SCAL_STRIDE);
     s.partials = cv.take<double>((size_t)s.ew_n);
-    s.counter = cv.take<unsigned>(1);
     s.delta = cv.take<float>((size_t)delta_cap);
     s.delta_cap = delta_cap;
   }
@@ -277,7 +280,8 @@ CellTask cell_task(const Slice& s, const float* img) {
   return CellTask{s.N, s.cl_begin, img, s.cells, s.cellsT};
 }
 FPTask fp_task(const Slice& s, float* out) {
-  return FPTask{s.N, s.T, s.mid, s.fp_begin, fp_ray_blocks(s.N), s.cells, s.cellsT, s.scfp, out};
+  return FPTask{s.N, s.T, s.mid, s.fp_begin, fp_ray_blocks(s.N), fp_chunks(s.N, s.fp_clen), s.fp_clen,
+                s.fs_begin, s.fpart, s.cells, s.cellsT, s.scfp, out};
 }
 
 template <class T, class F>
@@ -315,7 +319,7 @@ void build_tv_task_array(acu_batch* b) {
     t.uk = s.uk; t.ukp = s.ukp; t.dkx = s.dkx; t.dky = s.dky; t.bkx = s.bkx; t.bky = s.bky;
     t.ifkt = s.ifkt; t.iukt = s.iukt; t.r = s.r; t.p = s.p; t.irpt = s.irpt; t.Ap = s.Ap;
     t.f = s.f; t.fk = s.fk; t.ruk = s.ruk;
-    t.scal = s.scal; t.partials = s.partials; t.counter = s.counter;
+    t.scal = s.scal; t.partials = s.partials;
     t.delta_log = s.delta; t.delta_cap = s.delta_cap;
     return t;
   });
@@ -330,10 +334,7 @@ void ensure_tv(acu_batch* b, int delta_cap) {
   ACU_HIP_CHECK(hipMalloc(&b->tv_mem, dry.off + 256));
   Carver cv{static_cast<char*>(b->tv_mem), 0};
   layout_tv(b, cv, delta_cap);
-  for (auto& s : b->s) {
-    ACU_HIP_CHECK(hipMemset(s.counter, 0, sizeof(unsigned)));
-    ACU_HIP_CHECK(hipMemset(s.scal, 0, SCAL_STRIDE * sizeof(double)));
-  }
+  for (auto& s : b->s) ACU_HIP_CHECK(hipMemset(s.scal, 0, SCAL_STRIDE * sizeof(double)));
   b->tv_delta_cap = delta_cap;
   build_tv_tasks(b);
 }
@@ -349,6 +350,7 @@ Range rf_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->rf_total} : R
 Range bp_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->bp_total} : Range{b->s[sl].bp_begin, b->s[sl].bp_n}; }
 Range cl_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->cl_total} : Range{b->s[sl].cl_begin, b->s[sl].cl_n}; }
 Range fp_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->fp_total} : Range{b->s[sl].fp_begin, b->s[sl].fp_n}; }
+Range fs_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->fs_total} : Range{b->s[sl].fs_begin, b->s[sl].fs_n}; }
 Range ew_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->ew_total} : Range{b->s[sl].ew_begin, b->s[sl].ew_n}; }
 Range sw_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->sw_total} : Range{b->s[sl].sw_begin, b->s[sl].sw_n}; }
 
@@ -362,9 +364,9 @@ void iradon_seq(acu_batch* b, const DevArray<RFTask>& rf, const DevArray<BPTask>
 
 void radon_seq(acu_batch* b, const DevArray<CellTask>& cl, const DevArray<FPTask>& fp, int sl) {
   acu_ctx* c = b->ctx;
-  const Range r1 = cl_range(b, sl), r2 = fp_range(b, sl);
+  const Range r1 = cl_range(b, sl), r2 = fp_range(b, sl), r3 = fs_range(b, sl);
   timed(c, K_CELLS, [&] { launch_cells(cl.d, cl.n, r1.base, r1.n, c->stream); });
-  timed(c, K_FP, [&] { launch_forward(fp.d, fp.n, r2.base, r2.n, c->stream); });
+  timed(c, K_FP, [&] { launch_forward(fp.d, fp.n, r2.base, r2.n, r3.base, r3.n, c->stream); });
   check_launch();
 }
 
@@ -387,26 +389,31 @@ void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool reuse_ruk, bool 
   const Range ew = ew_range(b, sl), sw = sw_range(b, sl);
   const TVTask* tv = b->tv.d;
   const int nt = b->tv.n;
+  const int first = sl < 0 ? 0 : sl, count = sl < 0 ? nt : 1;  // finalize: one block per slice
   for (int ji = 0; ji < ninner; ++ji) {
     iradon_seq(b, b->rf_fk, b->bp_ifkt, sl);                       // :215
     if (!(reuse_ruk && ji == 0)) radon_seq(b, b->cl_uk, b->fp_ruk, sl);  // :228
     iradon_seq(b, b->rf_ruk, b->bp_iukt, sl);                      // :229
     launch_tv_residual(tv, nt, ew.base, ew.n, c->stream);          // :207-249
+    launch_tv_finalize(tv, first, count, 0, c->stream);
     check_launch();
     if (host_sync) (void)read_scalar(b, sl, 0);
     for (int j = 0; j < ncg; ++j) {                                // :252-303
       radon_seq(b, b->cl_p, b->fp_rp, sl);
       iradon_seq(b, b->rf_rp, b->bp_irpt, sl);
       launch_tv_apdot(tv, nt, ew.base, ew.n, j, c->stream);
+      launch_tv_finalize(tv, first, count, 64 + j, c->stream);
       check_launch();
       if (host_sync) (void)read_scalar(b, sl, 64 + j);
       launch_tv_axpy(tv, nt, ew.base, ew.n, j, c->stream);
+      launch_tv_finalize(tv, first, count, j + 1, c->stream);
       check_launch();
       if (host_sync) (void)read_scalar(b, sl, j + 1);
       launch_tv_pupdate(tv, nt, ew.base, ew.n, j, c->stream);
       check_launch();
     }
     launch_tv_shrink(tv, nt, ew.base, ew.n, c->stream);           // :305-333
+    launch_tv_finalize(tv, first, count, 127, c->stream);
     check_launch();
     if (host_sync) (void)read_scalar(b, sl, 127);
   }
@@ -501,8 +508,9 @@ void radon_dev(acu_batch* b, int sl, float* sino, const float* img) {
   {
     FPTask t = fp_task(s, sino);
     t.block_begin = 0;
+    t.sum_begin = 0;
     FPTask* d = put_adhoc(b, t);
-    timed(c, K_FP, [&] { launch_forward(d, 1, 0, s.fp_n, c->stream); });
+    timed(c, K_FP, [&] { launch_forward(d, 1, 0, s.fp_n, 0, s.fs_n, c->stream); });
   }
   check_launch();
 }
@@ -607,7 +615,10 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
   try {
     ACU_HIP_CHECK(hipSetDevice(ctx->device));
     b->s.resize(nslices);
-    int rf = 0, bp = 0, cl = 0, fp = 0, ew = 0, sw = 0;
+    int rf = 0, bp = 0, cl = 0, fp = 0, fs = 0, ew = 0, sw = 0;
+    // tuning: forward-projection sweep chunk length (steps), ACUMEN_FP_CHUNK
+    const char* ev = std::getenv("ACUMEN_FP_CHUNK");
+    const int fp_clen_env = ev ? std::atoi(ev) : 0;
     for (int i = 0; i < nslices; ++i) {
       auto& s = b->s[i];
       s.N = nimage[i]; s.T = ntheta[i]; s.wpu = wpu[i];
@@ -617,11 +628,13 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       s.rf_begin = rf; s.rf_n = acu::rf_tiles_m(s.N) * acu::rf_tiles_n(s.T); rf += s.rf_n;
       s.bp_begin = bp; s.bp_n = s.ntx * s.nty; bp += s.bp_n;
       s.cl_begin = cl; s.cl_n = acu::cell_blocks(s.N); cl += s.cl_n;
-      s.fp_begin = fp; s.fp_n = acu::fp_ray_blocks(s.N) * s.T; fp += s.fp_n;
+      s.fp_clen = fp_clen_env > 0 ? fp_clen_env : acu::FP_CHUNK_DEFAULT;
+      s.fp_begin = fp; s.fp_n = acu::fp_ray_blocks(s.N) * s.T * acu::fp_chunks(s.N, s.fp_clen); fp += s.fp_n;
+      s.fs_begin = fs; s.fs_n = acu::fp_sum_blocks(s.N, s.T); fs += s.fs_n;
       s.ew_begin = ew; s.ew_n = acu::ew_blocks((size_t)s.N * s.N); ew += s.ew_n;
       s.sw_begin = sw; s.sw_n = acu::ew_blocks((size_t)s.N * s.T); sw += s.sw_n;
     }
-    b->rf_total = rf; b->bp_total = bp; b->cl_total = cl; b->fp_total = fp;
+    b->rf_total = rf; b->bp_total = bp; b->cl_total = cl; b->fp_total = fp; b->fs_total = fs;
     b->ew_total = ew; b->sw_total = sw;
     acu::Carver dry;
     acu::layout_ops(b.get(), dry);

@@ -25,7 +25,15 @@
 //    lanes are skewed along their rays (see ray_sum) so that one gather of a
 //    wave stays within about one row of that copy;
 //  * each wave skips the steps before the first and after the last cell
-//    any of its rays can use (the skipped samples are all rejected ones).
+//    any of its rays can use (the skipped samples are all rejected ones);
+//  * workgroups are ordered (chunk, ray block, angle) and remapped so that
+//    the ones an XCD runs together are neighbouring angles of one ray block:
+//    their rays sweep nearly the same cells, so the gathers hit that XCD's
+//    L2 (measured: 11.7 ms -> 6.5 ms per 2049^2 x 1500 launch against the
+//    hardware's round-robin order).  A ray's sweep may be cut into chunks of
+//    clen steps (tuning knob ACUMEN_FP_CHUNK, default: one chunk); each
+//    chunk writes its own partial sum and sum_kernel adds a ray's chunks in
+//    step order.
 #include "acu_internal.hpp"
 
 namespace acu {
@@ -111,6 +119,7 @@ template <bool XMAJOR>
 __device__ __forceinline__ float ray_sum(const ACU_GLOBAL float4* __restrict__ cells, int xs, int xe,
                                          int off, int k0, int k1, float s, float c, float sy,
                                          float cy, float m, int N) {
+  // lanes sum steps x in [xs, xe]; iteration k visits x = k - off
 #pragma clang fp contract(off)
   const int P = N + 1;
   const float fN = (float)N;
@@ -176,7 +185,13 @@ __global__ __launch_bounds__(FP_THREADS) void fp_kernel(const FPTask* __restrict
   const int N = tk.N;
   const float m = tk.mid;
   const int local = gb - tk.block_begin;
-  const int ang = local % tk.T, rb = local / tk.T;
+  // chunk-major, then ray block, then angle: the workgroups one XCD runs
+  // together are neighbouring angles of one ray block (and one chunk)
+  const int chunk = local / (tk.nrb * tk.T);
+  const int rem = local - chunk * (tk.nrb * tk.T);
+  const int rb = rem / tk.T;
+  const int ang = rem - rb * tk.T;
+  const int cx0 = chunk * tk.clen + 1, cx1 = min(N, cx0 + tk.clen - 1);  // 1-based steps
   const int ray0 = rb * FP_THREADS + (threadIdx.x & ~63);
   if (ray0 >= N) return;  // whole wave beyond the detector
   const int lane = threadIdx.x & 63;
@@ -198,8 +213,8 @@ __global__ __launch_bounds__(FP_THREADS) void fp_kernel(const FPTask* __restrict
   // 1-based step x = u + m, clipped to [1, N] with one step of margin
   int xs = 1, xe = 0;
   if (u0 <= u1) {
-    xs = max(1, (int)floorf(fmaxf(u0 + m, -1e9f)) - 1);
-    xe = min(N, (int)ceilf(fminf(u1 + m, 1e9f)) + 1);
+    xs = max(cx0, (int)floorf(fmaxf(u0 + m, -1e9f)) - 1);
+    xe = min(cx1, (int)ceilf(fminf(u1 + m, 1e9f)) + 1);
   }
   xs = __builtin_amdgcn_readfirstlane(xs);
   xe = __builtin_amdgcn_readfirstlane(xe);
@@ -213,12 +228,32 @@ __global__ __launch_bounds__(FP_THREADS) void fp_kernel(const FPTask* __restrict
   if (xs <= xe)
     acc = xmajor ? ray_sum<true>(as_global(tk.cells), xs, xe, off, k0, k1, s, c, sy, cy, m, N)
                  : ray_sum<false>(as_global(tk.cellsT), xs, xe, off, k0, k1, s, c, sy, cy, m, N);
-  if (ray < N) tk.sino[(size_t)ang * N + ray] = acc / (float)N;
+  if (ray < N) tk.partial[((size_t)chunk * tk.T + ang) * N + ray] = acc;
+}
+
+// sino(y, t) = (sum over chunks, in step order, of the partial sums) / N
+__global__ __launch_bounds__(256) void sum_kernel(const FPTask* __restrict__ tasks, int ntasks, int base) {
+  const int gb = base + blockIdx.x;
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].sum_begin <= gb) lo = mid; else hi = mid - 1;
+  }
+  const FPTask& tk = tasks[lo];
+  const size_t NT = (size_t)tk.N * tk.T;
+  const size_t e = (size_t)(gb - tk.sum_begin) * 256 + threadIdx.x;
+  if (e >= NT) return;
+  const float* p = tk.partial + e;
+  float acc = 0.0f;
+  for (int c = 0; c < tk.nchunk; ++c) acc += p[(size_t)c * NT];
+  tk.sino[e] = acc / (float)tk.N;
 }
 
 }  // namespace
 
 int fp_ray_blocks(int N) { return (N + FP_THREADS - 1) / FP_THREADS; }
+int fp_chunks(int N, int clen) { return (N + clen - 1) / clen; }
+int fp_sum_blocks(int N, int T) { return (int)(((size_t)N * T + 255) / 256); }
 int cell_blocks(int N) { const int nb = (N + 1 + CT - 1) / CT; return nb * nb; }
 
 void launch_cells(const CellTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s) {
@@ -226,10 +261,12 @@ void launch_cells(const CellTask* d_tasks, int ntasks, int base, int blocks, hip
   hipLaunchKernelGGL(cells_kernel, dim3(blocks), dim3(256), 0, s, d_tasks, ntasks, base);
 }
 
-void launch_forward(const FPTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s) {
+void launch_forward(const FPTask* d_tasks, int ntasks, int base, int blocks, int sum_base,
+                    int sum_blocks, hipStream_t s) {
   if (blocks <= 0) return;
   hipLaunchKernelGGL(fp_kernel, dim3(xcd_grid(blocks)), dim3(FP_THREADS), 0, s, d_tasks, ntasks, base,
                      blocks);
+  hipLaunchKernelGGL(sum_kernel, dim3(sum_blocks), dim3(256), 0, s, d_tasks, ntasks, sum_base);
 }
 
 }  // namespace acu

@@ -4,14 +4,15 @@
 // The reference runs every loop nest as its own kernel and returns each FP32
 // reduction to the host (12 host round trips per inner iteration at ncg=5).
 // Here each kernel fuses an update with the reduction that consumes it, the
-// reductions are FP64 (wavefront shuffles, then LDS, then the last block of
-// the grid sums the block partials in a fixed order), and every scalar
+// reductions are FP64 (wavefront shuffles, then LDS, then one workgroup per
+// slice sums the block partials in a fixed order), and every scalar
 // (rsold, p.Ap, rsnew, alpha, beta, Err_diff) lives in device memory, so an
 // outer iteration needs no host synchronisation and can be replayed from a
 // HIP graph.  All kernels are batched: blocks are flattened over slices.
 //
 // Scalar slots per slice (doubles): rs[j] at j (j = 0..ncg), pAp[j] at
-// 64+j, outer-iteration counter at 126, Err_diff at 127.
+// 64+j, outer-iteration counter at 126, Err_diff at 127.  Every reduction
+// kernel writes block partials and is followed by finalize_kernel.
 #include "acu_tv.hpp"
 
 namespace acu {
@@ -44,12 +45,12 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block-reduce v, write the block partial, and let the last block of this
-// slice sum all partials in index order into *out.  Deterministic.
-__device__ void reduce_to_scalar(double v, double* partials, unsigned* counter, int nblocks,
-                                 int lblock, double* out) {
+// Block-reduce v and write this block's partial; finalize_kernel (launched
+// next on the same stream) sums a slice's partials in index order.  No
+// device-scope fence per block: on a multi-XCD part each such fence writes
+// back the L2, which made these kernels ~10x slower than their traffic.
+__device__ void reduce_to_partial(double v, double* partials, int lblock) {
   __shared__ double wsum[NT_ / 64];
-  __shared__ bool last;
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) wsum[wave] = v;
@@ -59,28 +60,6 @@ __device__ void reduce_to_scalar(double v, double* partials, unsigned* counter, 
 #pragma unroll
     for (int w = 0; w < NT_ / 64; ++w) b += wsum[w];
     partials[lblock] = b;
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    last = (prev == (unsigned)nblocks - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // fixed-order sum of the partials: thread t sums a contiguous chunk
-  const int chunk = (nblocks + NT_ - 1) / NT_;
-  double s = 0.0;
-  const int b0 = threadIdx.x * chunk;
-  for (int b = b0; b < min(nblocks, b0 + chunk); ++b) s += partials[b];
-  s = wave_sum(s);
-  __syncthreads();
-  if (lane == 0) wsum[wave] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double tot = 0.0;
-#pragma unroll
-    for (int w = 0; w < NT_ / 64; ++w) tot += wsum[w];
-    *out = tot;
-    *counter = 0u;  // ready for the next reduction on this slice
   }
 }
 
@@ -126,7 +105,7 @@ __global__ __launch_bounds__(NT_) void residual_kernel(const TVTask* __restrict_
       acc += (double)rv * (double)rv;
     }
   }
-  reduce_to_scalar(acc, tk.partials, tk.counter, tk.nb, lb, tk.scal + 0);
+  reduce_to_partial(acc, tk.partials, lb);
 }
 
 // K12 + K13 (tv_hpc.F90:257-274): Ap = mu*irpt + lambda*L(p); pAp[j] = p.Ap
@@ -147,7 +126,7 @@ __global__ __launch_bounds__(NT_) void apdot_kernel(const TVTask* __restrict__ t
       acc += (double)tk.p[e] * (double)a;
     }
   }
-  reduce_to_scalar(acc, tk.partials, tk.counter, tk.nb, lb, tk.scal + 64 + j);
+  reduce_to_partial(acc, tk.partials, lb);
 }
 
 // K14 + K15 (tv_hpc.F90:277-292): alpha = rs[j]/pAp[j] (FP64);
@@ -172,7 +151,7 @@ __global__ __launch_bounds__(NT_) void axpy_kernel(const TVTask* __restrict__ ta
       acc += (double)rv * (double)rv;
     }
   }
-  reduce_to_scalar(acc, tk.partials, tk.counter, tk.nb, lb, tk.scal + j + 1);
+  reduce_to_partial(acc, tk.partials, lb);
 }
 
 // K16 (tv_hpc.F90:294-299): p = r + (rs[j+1]/rs[j]) p
@@ -187,6 +166,29 @@ __global__ __launch_bounds__(NT_) void pupdate_kernel(const TVTask* __restrict__
   for (int it = 0; it < IT; ++it) {
     const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
     if (e < NN) tk.p[e] = tk.r[e] + beta * tk.p[e];
+  }
+}
+
+// One workgroup per slice: scal[slot] = sum of the slice's block partials,
+// in a fixed order (thread t sums a contiguous range, then a tree).
+__global__ __launch_bounds__(NT_) void finalize_kernel(const TVTask* __restrict__ tasks, int first,
+                                                       int slot) {
+  const TVTask& tk = tasks[first + blockIdx.x];
+  __shared__ double wsum[NT_ / 64];
+  const int nb = tk.nb;
+  const int chunk = (nb + NT_ - 1) / NT_;
+  const int b0 = threadIdx.x * chunk;
+  double s = 0.0;
+  for (int b = b0; b < min(nb, b0 + chunk); ++b) s += tk.partials[b];
+  s = wave_sum(s);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wsum[wave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT_ / 64; ++w) tot += wsum[w];
+    tk.scal[slot] = tot;
   }
 }
 
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(NT_) void shrink_kernel(const TVTask* __restrict__ 
       acc += (double)fabsf(u - tk.ukp[e]);
     }
   }
-  reduce_to_scalar(acc, tk.partials, tk.counter, tk.nb, lb, tk.scal + 127);
+  reduce_to_partial(acc, tk.partials, lb);
 }
 
 // K19 (tv_hpc.F90:341-346): fk = (fk + f) - ruk, plus the delta log entry.
@@ -278,6 +280,9 @@ __global__ __launch_bounds__(NT_) void fkinit_kernel(const TVTask* __restrict__ 
 
 int ew_blocks(size_t n) { return (int)((n + PER_BLOCK - 1) / PER_BLOCK); }
 
+void launch_tv_finalize(const TVTask* t, int first, int count, int slot, hipStream_t s) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(count), dim3(NT_), 0, s, t, first, slot);
+}
 void launch_tv_residual(const TVTask* t, int n, int base, int blocks, hipStream_t s) {
   hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base);
 }

@@ -1,0 +1,139 @@
+! acumen_io.F90 -- the reference drivers' on-disk contract without MPI-IO.
+!
+! The reference opens every file with MPI_FILE_OPEN and an MPI_REAL4 view, so
+! all offsets are FP32 element counts (e.g. tomography/tv_hpc.F90:463-476,
+! 529-531, 612, 624-626).  Here the same files are read and written with
+! Fortran stream access at byte position 4*offset + 1.
+module acumen_io
+  implicit none
+  integer, parameter :: IO_WARN_UNIT = 6
+
+contains
+
+  function rec_dir(root_dir, exp_name) result(d)
+    character(len=*), intent(in) :: root_dir, exp_name
+    character(len=:), allocatable :: d
+    d = trim(root_dir) // "/" // trim(exp_name) // "/reconstruction/"
+  end function rec_dir
+
+  function numbered(prefix, js) result(name)
+    character(len=*), intent(in) :: prefix
+    integer, intent(in) :: js
+    character(len=:), allocatable :: name
+    character(len=32) :: c
+    write(c, *) js
+    name = prefix // trim(adjustl(c)) // ".dat"
+  end function numbered
+
+  ! FP32 element at `offset` of a file; on a short file the value is left
+  ! unchanged, as an MPI_FILE_READ_AT that reads nothing leaves its buffer
+  ! (the reference's fileinfo reads rely on this, fbp_hpc.F90:270-275).
+  subroutine read_real_at(path, offset, v)
+    character(len=*), intent(in) :: path
+    integer(kind=8), intent(in) :: offset
+    real, intent(inout) :: v
+    integer :: u, ios
+    real :: t
+    open(newunit=u, file=path, access='stream', form='unformatted', status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      write(IO_WARN_UNIT, *) "acumen: cannot open ", trim(path)
+      error stop 3
+    end if
+    read(u, pos=4_8 * offset + 1_8, iostat=ios) t
+    if (ios == 0) then
+      v = t
+    else
+      write(IO_WARN_UNIT, *) "acumen: warning: ", trim(path), " has no element at offset", offset, &
+        "; value left unchanged"
+    end if
+    close(u)
+  end subroutine read_real_at
+
+  subroutine read_reals_at(path, offset, n, a)
+    character(len=*), intent(in) :: path
+    integer(kind=8), intent(in) :: offset
+    integer, intent(in) :: n
+    real, intent(out) :: a(n)
+    integer :: u, ios
+    open(newunit=u, file=path, access='stream', form='unformatted', status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      write(IO_WARN_UNIT, *) "acumen: cannot open ", trim(path)
+      error stop 3
+    end if
+    read(u, pos=4_8 * offset + 1_8, iostat=ios) a
+    if (ios /= 0) then
+      write(IO_WARN_UNIT, *) "acumen: short read of", n, "values at offset", offset, " in ", trim(path)
+      error stop 3
+    end if
+    close(u)
+  end subroutine read_reals_at
+
+  subroutine write_reals_at(path, offset, n, a)
+    character(len=*), intent(in) :: path
+    integer(kind=8), intent(in) :: offset
+    integer, intent(in) :: n
+    real, intent(in) :: a(n)
+    integer :: u, ios
+    open(newunit=u, file=path, access='stream', form='unformatted', status='unknown', action='readwrite', &
+         iostat=ios)
+    if (ios /= 0) then
+      write(IO_WARN_UNIT, *) "acumen: cannot open ", trim(path), " for writing"
+      error stop 3
+    end if
+    write(u, pos=4_8 * offset + 1_8, iostat=ios) a
+    if (ios /= 0) then
+      write(IO_WARN_UNIT, *) "acumen: write failed at offset", offset, " in ", trim(path)
+      error stop 3
+    end if
+    close(u)
+  end subroutine write_reals_at
+
+  subroutine write_real_at(path, offset, v)
+    character(len=*), intent(in) :: path
+    integer(kind=8), intent(in) :: offset
+    real, intent(in) :: v
+    real :: a(1)
+    a(1) = v
+    call write_reals_at(path, offset, 1, a)
+  end subroutine write_real_at
+
+  ! input_parm.txt: nsub, nsub_t, ninner, nouter, ncg, lambda, mu, wpu
+  ! (fbp_hpc.F90:208-218, tv_hpc_slice.F90:419-429)
+  subroutine read_input_parm(path, nsub, nsub_t, ninner, nouter, ncg, lambda, mu, wpu)
+    character(len=*), intent(in) :: path
+    integer, intent(out) :: nsub, nsub_t, ninner, nouter, ncg, wpu
+    real, intent(out) :: lambda, mu
+    integer :: u, ios
+    open(newunit=u, file=path, status='old', action='read', position='rewind', iostat=ios)
+    if (ios /= 0) then
+      write(IO_WARN_UNIT, *) "acumen: cannot open ", trim(path)
+      error stop 3
+    end if
+    read(u, *) nsub
+    read(u, *) nsub_t
+    read(u, *) ninner
+    read(u, *) nouter
+    read(u, *) ncg
+    read(u, *) lambda
+    read(u, *) mu
+    read(u, *) wpu
+    close(u)
+  end subroutine read_input_parm
+
+  ! Angle window [argmin |theta|, argmin |theta - 180|] (first minima),
+  ! tv_hpc.F90:546-547
+  subroutine angle_window(f_theta, jtheta_s, jtheta_e)
+    real, intent(in) :: f_theta(:)
+    integer, intent(out) :: jtheta_s, jtheta_e
+    jtheta_s = minloc(abs(f_theta), DIM=1)
+    jtheta_e = minloc(abs(f_theta - 180.0), DIM=1)
+  end subroutine angle_window
+
+  function wall_seconds() result(t)
+    real(kind=8) :: t
+    integer(kind=8) :: c, r
+    call system_clock(c, r)
+    t = real(c, 8) / real(r, 8)
+  end function wall_seconds
+
+end module acumen_io

@@ -1,0 +1,122 @@
+"""The reference drivers' on-disk contract.
+
+Directory ``$ROOT_CT_DIR/$EXP_CT_NAME/reconstruction/`` holding (all FP32,
+native byte order, first index fastest):
+
+* ``fileinfo_projections.dat`` -- header ``[nfiles, nimagef, n_files_sino,
+  n_files_rec]`` as written by tomography/fits_to_hpc.F90:257-266.  fbp_hpc
+  and tv_hpc_slice overwrite offsets 2-9 with ``nsub, nsub_t, ninner, nouter,
+  ncg, lambda, mu, wpu`` and then read the file counts from offsets 10-11
+  (fbp_hpc.F90:220-275), so a dataset meant for every driver carries a
+  12-value header whose offsets 2-3 and 10-11 both hold the file counts.
+* ``theta.dat`` -- ``nfiles`` angles in degrees.
+* ``sinograms{js}.dat`` -- per volume slice ``(nimagef+add_Nx) x nfiles``,
+  detector fastest (projections_to_sinogram_hpc.F90:479-487).
+* ``fbp{js}.dat``, ``fbp_tv{js}.dat`` (``fbp_sub``/``fbp_sub_tv`` when
+  sub-sampled), ``fbp_tv_slice.dat``, ``fbp_sub_slice.dat`` -- ``N x N``
+  images, x fastest.
+* ``../input_parm.txt`` -- 8 list-directed lines
+  ``nsub, nsub_t, ninner, nouter, ncg, lambda, mu, wpu``.
+
+Slice ``s`` (1-based) of ``nimagef`` lives in file ``js = ceil(s / r)`` at
+element offset ``(s - floor((js-1) r) - 1) * rowlen`` with
+``r = nimagef / nfiles_of_that_kind`` (tv_hpc.F90:609-610, 622-623).
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class Dataset:
+    root: str
+    name: str
+
+    @property
+    def rec(self) -> str:
+        return os.path.join(self.root, self.name, "reconstruction")
+
+    def path(self, fname: str) -> str:
+        return os.path.join(self.rec, fname)
+
+
+def _f32(x) -> np.ndarray:
+    return np.asarray(x, dtype=np.float32)
+
+
+def slice_location(s: int, nimagef: int, nfiles_kind: int, rowlen: int):
+    """(js, element offset) of 1-based slice s, as the reference computes it
+    in FP32 (tv_hpc.F90:600-610)."""
+    r = np.float32(nimagef) / np.float32(nfiles_kind)
+    js = int(math.ceil(np.float32(s) / r))
+    off = (s - int(math.floor(np.float32(js - 1) * r)) - 1) * rowlen
+    return js, off
+
+
+def write_dataset(ds: Dataset, theta_deg, sinograms, n_files_sino: int = 1,
+                  n_files_rec: int = 1, parm=None) -> None:
+    """sinograms: array (nslices, nfiles, nimagef+add_Nx) -- one sinogram per
+    volume slice, angle-major rows (detector fastest).  nslices must equal
+    nimagef (the reference reconstructs one slice per detector row)."""
+    os.makedirs(ds.rec, exist_ok=True)
+    theta = _f32(theta_deg)
+    nfiles = theta.size
+    sino = _f32(sinograms)
+    nslices, nf, width = sino.shape
+    assert nf == nfiles
+    nimagef = nslices
+    add = 1 if nimagef % 2 == 0 else 0
+    assert width == nimagef + add, "sinogram rows are nimagef+add_Nx wide"
+    hdr = np.zeros(12, np.float32)
+    hdr[:4] = [nfiles, nimagef, n_files_sino, n_files_rec]
+    hdr[10:12] = [n_files_sino, n_files_rec]
+    hdr.tofile(ds.path("fileinfo_projections.dat"))
+    theta.tofile(ds.path("theta.dat"))
+    rowlen = width * nfiles
+    files = {}
+    for s in range(1, nslices + 1):
+        js, off = slice_location(s, nimagef, n_files_sino, rowlen)
+        files.setdefault(js, []).append((off, sino[s - 1]))
+    for js, parts in files.items():
+        size = max(o for o, _ in parts) + rowlen
+        buf = np.zeros(size, np.float32)
+        for o, a in parts:
+            buf[o:o + rowlen] = a.ravel()
+        buf.tofile(ds.path(f"sinograms{js}.dat"))
+    if parm is not None:
+        write_input_parm(ds, **parm)
+
+
+def write_input_parm(ds: Dataset, nsub=1, nsub_t=1, ninner=1, nouter=1, ncg=5,
+                     lam=0.1, mu=0.05, wpu=4) -> None:
+    with open(os.path.join(ds.root, ds.name, "input_parm.txt"), "w") as fh:
+        for v in (nsub, nsub_t, ninner, nouter, ncg, lam, mu, wpu):
+            fh.write(f"{v}\n")
+
+
+def read_images(ds: Dataset, prefix: str, nimagef: int, nimage: int,
+                n_files_rec: int = 1) -> np.ndarray:
+    """All slices of fbp/fbp_tv{js}.dat as (nimagef, nimage, nimage)."""
+    out = np.zeros((nimagef, nimage, nimage), np.float32)
+    cache = {}
+    for s in range(1, nimagef + 1):
+        js, off = slice_location(s, nimagef, n_files_rec, nimage * nimage)
+        if js not in cache:
+            cache[js] = np.fromfile(ds.path(f"{prefix}{js}.dat"), np.float32)
+        out[s - 1] = cache[js][off:off + nimage * nimage].reshape(nimage, nimage)
+    return out
+
+
+def read_image(ds: Dataset, fname: str, nimage: int) -> np.ndarray:
+    return np.fromfile(ds.path(fname), np.float32)[:nimage * nimage].reshape(nimage, nimage)
+
+
+def angle_window(theta_deg):
+    """0-based (start, end) of the reference's window: first argmin |theta|
+    and first argmin |theta - 180| (tv_hpc.F90:546-547)."""
+    th = _f32(theta_deg)
+    return int(np.argmin(np.abs(th))), int(np.argmin(np.abs(th - np.float32(180.0))))

@@ -1,0 +1,105 @@
+"""The Fortran drop-in drivers (fbp_hip, tv_hip, tv_hip_slice) on a synthetic
+volume: same results as the C ABI path, bit for bit, and parity with the
+oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from acumen_amd import ACU_IMG_FBP, ACU_IMG_TV, ACU_TV_GRAPH, Batch, synth
+from acumen_amd import io as aio
+from helpers import disk_phantom, rel
+from oracle import ref
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "acumen_amd", "fortran", "bin")
+NIMAGEF, NFILES = 32, 37          # 32 slices, 37 angles 0..180 step 5 -> N = 33
+MU, LAM, NOUTER, NCG, WPU = 0.05, 0.1, 2, 3, 4
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def volume(tmp_path_factory):
+    root = tmp_path_factory.mktemp("ct")
+    ds = aio.Dataset(str(root), "synthetic")
+    th = np.arange(NFILES, dtype=np.float32) * np.float32(5.0)
+    N = NIMAGEF + 1
+    sinos = np.stack([ref.radon(disk_phantom(N, 4, 300 + s), th, "f64").astype(np.float32)
+                      for s in range(NIMAGEF)])
+    aio.write_dataset(ds, th, sinos, parm=dict(nsub=1, nsub_t=1, ninner=1, nouter=NOUTER, ncg=NCG,
+                                               lam=LAM, mu=MU, wpu=WPU))
+    return ds, th, sinos
+
+
+def run(exe, ds, extra=None):
+    env = dict(os.environ, ROOT_CT_DIR=ds.root, EXP_CT_NAME=ds.name)
+    env.update(extra or {})
+    path = os.path.join(BIN, exe)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (python -m acumen_amd.fortran_build)")
+    say("driver", exe)
+    r = subprocess.run([path], capture_output=True, text=True, env=env, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_fbp_driver(ctx, volume):
+    ds, th, sinos = volume
+    N = NIMAGEF + 1
+    out = run("fbp_hip", ds, {"ACUMEN_BATCH": "10"})
+    assert "Reconstruction with Resolution:" in out
+    imgs = aio.read_images(ds, "fbp", NIMAGEF, N)
+    with Batch(ctx, [(N, NFILES, WPU, th)]) as b:
+        for s in (0, 13, NIMAGEF - 1):
+            b.set_sino(0, sinos[s])
+            b.fbp()
+            assert np.array_equal(imgs[s], b.get_image(0, ACU_IMG_FBP))
+            assert rel(imgs[s], ref.iradon(sinos[s], th, WPU, "f32")) < 1e-5
+
+
+def test_tv_slice_driver(ctx, volume):
+    ds, th, sinos = volume
+    N = NIMAGEF + 1
+    run("tv_hip_slice", ds)
+    tv = aio.read_image(ds, "fbp_tv_slice.dat", N)
+    fbp = aio.read_image(ds, "fbp_sub_slice.dat", N)
+    s = N // 2 - 1  # job_id = nimage/2 (1-based)
+    with Batch(ctx, [(N, NFILES, WPU, th)]) as b:
+        b.set_sino(0, sinos[s])
+        b.tvopt(MU, LAM, NOUTER, 1, NCG, ACU_TV_GRAPH)
+        assert np.array_equal(tv, b.get_image(0, ACU_IMG_TV))
+        assert np.array_equal(fbp, b.get_image(0, ACU_IMG_FBP))
+    tv64, _, _ = ref.tvopt(sinos[s], th, WPU, MU, LAM, NOUTER, 1, NCG, "f64")
+    assert rel(tv, tv64) < 1e-4
+
+
+def tv_env(nsub, nsub_t):
+    return {"Sub_Sample_Radial": str(nsub), "Sub_Sample_Theta": str(nsub_t), "Ninner_Loops": "1",
+            "Nouter_Loops": str(NOUTER), "Opt_ncg": str(NCG), "Opt_lambda": str(LAM),
+            "Opt_mu": str(MU), "Radon_wpu": str(WPU), "ACUMEN_BATCH": "12"}
+
+
+@pytest.mark.parametrize("nsub,nsub_t", [(1, 1), (2, 2)])
+def test_tv_volume_driver(ctx, volume, nsub, nsub_t):
+    ds, th, sinos = volume
+    run("tv_hip", ds, tv_env(nsub, nsub_t))
+    width = NIMAGEF + 1
+    N = (width - 1) // nsub + 1
+    T = NFILES // nsub_t
+    thw = th[::nsub_t][:T]
+    pre = "fbp" if nsub + nsub_t == 2 else "fbp_sub"
+    tvs = aio.read_images(ds, pre + "_tv", NIMAGEF, N)
+    fbps = aio.read_images(ds, pre, NIMAGEF, N)
+    with Batch(ctx, [(N, T, WPU, thw)]) as b:
+        for s in (0, 17, NIMAGEF - 1):
+            f = sinos[s][::nsub_t][:T, ::nsub][:, :N]
+            b.set_sino(0, f)
+            b.tvopt(MU, LAM, NOUTER, 1, NCG, ACU_TV_GRAPH)
+            assert np.array_equal(tvs[s], b.get_image(0, ACU_IMG_TV))
+            assert np.array_equal(fbps[s], b.get_image(0, ACU_IMG_FBP))
