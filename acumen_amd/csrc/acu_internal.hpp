@@ -67,6 +67,11 @@ constexpr int BP_W = 76;         // detector bins staged per angle (>= ceil(hypo
 constexpr int FP_THREADS = 256;  // rays per workgroup (one angle per workgroup)
 constexpr int FP_CHUNK_DEFAULT = 1 << 20;  // steps per sweep chunk (default: the whole ray)
 
+// ---- forward projection with LDS-staged boxes (fpl.hip) ----
+constexpr int FPL_U = 32;          // steps per staged chunk
+constexpr int FPL_PITCH = 111;     // LDS row pitch of the box (odd; pitch+1 <= 255 dwords)
+constexpr int FPL_CAP = 111 * 110; // floats of the LDS box (~48 KB): 111 x 110 pixels
+
 // ---- ramp filter GEMM (filter.hip) ----
 constexpr int RF_BM = 64;   // detector rows per tile
 constexpr int RF_BN = 64;   // angles per tile
@@ -105,6 +110,18 @@ struct FPTask {
   float* sino;           // N*T out
 };
 
+struct FPLTask {
+  int N, T;
+  float mid;
+  int block_begin;       // first global block index of this slice
+  int al;                // angles per lane (1, 2 or 4, the same for every slice of a launch)
+  int nab;               // angle blocks = ceil(T / (4*al)); block = rb * nab + ab
+  const float* img;      // N*N image (raw pixels, staged in LDS)
+  const float4* cells;   // x-major cells of the same image (fallback for oversized boxes)
+  const float2* sc;      // per-angle (sin, cos) for projection
+  float* sino;           // N*T out
+};
+
 struct RFTask {
   int N, T;
   int tile_begin;
@@ -130,6 +147,8 @@ void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, 
 void launch_cells(const CellTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
 void launch_forward(const FPTask* d_tasks, int ntasks, int base, int blocks, int sum_base,
                     int sum_blocks, hipStream_t s);
+void launch_forward_lds(const FPLTask* d_tasks, int ntasks, int base, int blocks, int al,
+                        hipStream_t s);
 
 // Host-side helpers.
 int bp_tiles_x(int N);
@@ -139,6 +158,8 @@ int rf_tiles_n(int T);
 int fp_ray_blocks(int N);
 int fp_chunks(int N, int clen);
 int fp_sum_blocks(int N, int T);
+int fpl_angles_per_lane(int N, int T);
+int fpl_blocks(int N, int T, int al);
 int cell_blocks(int N);
 
 }  // namespace acu

@@ -147,6 +147,7 @@ struct Slice {
   float4 *cells = nullptr, *cellsT = nullptr;
   float* fpart = nullptr;  // per-chunk partial ray sums of the forward projector
   int fp_clen = FP_CHUNK_DEFAULT;
+  int fpl_al = 1;
   // TV arena
   float *uk = nullptr, *ukp = nullptr, *dkx = nullptr, *dky = nullptr, *bkx = nullptr,
         *bky = nullptr, *ifkt = nullptr, *iukt = nullptr, *r = nullptr, *p = nullptr,
@@ -157,7 +158,7 @@ struct Slice {
   bool fbp_valid = false, tv_valid = false;
   // global block ranges in the batched launches
   int rf_begin = 0, rf_n = 0, bp_begin = 0, bp_n = 0, cl_begin = 0, cl_n = 0,
-      fp_begin = 0, fp_n = 0, fs_begin = 0, fs_n = 0, ew_begin = 0, ew_n = 0, sw_begin = 0, sw_n = 0;
+      fp_begin = 0, fp_n = 0, fs_begin = 0, fs_n = 0, fl_begin = 0, fl_n = 0, ew_begin = 0, ew_n = 0, sw_begin = 0, sw_n = 0;
 };
 
 }  // namespace
@@ -169,13 +170,16 @@ struct acu_batch {
   void* ops_mem = nullptr;
   void* tv_mem = nullptr;
   int tv_delta_cap = 0;
-  int rf_total = 0, bp_total = 0, cl_total = 0, fp_total = 0, fs_total = 0, ew_total = 0,
-      sw_total = 0;
+  int rf_total = 0, bp_total = 0, cl_total = 0, fp_total = 0, fs_total = 0, fl_total = 0,
+      ew_total = 0, sw_total = 0;
+  bool fp_lds = true;  // forward projection path: LDS boxes (fpl.hip) or global gathers (fp.hip)
+  int fpl_al = 4;      // angles per lane of the LDS forward projector, batch-wide
   // task sets for the fixed operand combinations of FBP / TV
   acu::DevArray<acu::RFTask> rf_f, rf_fk, rf_ruk, rf_rp;
   acu::DevArray<acu::BPTask> bp_fbp, bp_ifkt, bp_iukt, bp_irpt;
   acu::DevArray<acu::CellTask> cl_uk, cl_p;
   acu::DevArray<acu::FPTask> fp_ruk, fp_rp;
+  acu::DevArray<acu::FPLTask> fl_ruk, fl_rp;
   acu::DevArray<acu::TVTask> tv;
   float tv_mu = 0.f, tv_lambda = 0.f;
   // single-slot task buffer for calls on caller-provided device pointers
@@ -189,6 +193,7 @@ struct acu_batch {
     rf_f.release(); rf_fk.release(); rf_ruk.release(); rf_rp.release();
     bp_fbp.release(); bp_ifkt.release(); bp_iukt.release(); bp_irpt.release();
     cl_uk.release(); cl_p.release(); fp_ruk.release(); fp_rp.release(); tv.release();
+    fl_ruk.release(); fl_rp.release();
     if (adhoc) hipFree(adhoc);
     if (tv_mem) hipFree(tv_mem);
     if (ops_mem) hipFree(ops_mem);
@@ -276,6 +281,10 @@ RFTask rf_task(const Slice& s, const float* in, float* out) {
 BPTask bp_task(const Slice& s, const float* in, float* out) {
   return BPTask{s.N, s.T, s.mid, s.bp_begin, s.ntx, s.nty, in, s.cmask, s.scbp, s.tact, out};
 }
+FPLTask fpl_task(const Slice& s, const float* img, float* out) {
+  return FPLTask{s.N, s.T, s.mid, s.fl_begin, s.fpl_al, (s.T + 4 * s.fpl_al - 1) / (4 * s.fpl_al),
+                 img, s.cells, s.scfp, out};
+}
 CellTask cell_task(const Slice& s, const float* img) {
   return CellTask{s.N, s.cl_begin, img, s.cells, s.cellsT};
 }
@@ -307,6 +316,8 @@ void build_tv_tasks(acu_batch* b) {
   build(b, b->cl_p, [](Slice& s) { return cell_task(s, s.p); });
   build(b, b->fp_ruk, [](Slice& s) { return fp_task(s, s.ruk); });
   build(b, b->fp_rp, [](Slice& s) { return fp_task(s, s.rp); });
+  build(b, b->fl_ruk, [](Slice& s) { return fpl_task(s, s.uk, s.ruk); });
+  build(b, b->fl_rp, [](Slice& s) { return fpl_task(s, s.p, s.rp); });
 }
 
 void build_tv_task_array(acu_batch* b) {
@@ -351,6 +362,7 @@ Range bp_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->bp_total} : R
 Range cl_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->cl_total} : Range{b->s[sl].cl_begin, b->s[sl].cl_n}; }
 Range fp_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->fp_total} : Range{b->s[sl].fp_begin, b->s[sl].fp_n}; }
 Range fs_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->fs_total} : Range{b->s[sl].fs_begin, b->s[sl].fs_n}; }
+Range fl_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->fl_total} : Range{b->s[sl].fl_begin, b->s[sl].fl_n}; }
 Range ew_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->ew_total} : Range{b->s[sl].ew_begin, b->s[sl].ew_n}; }
 Range sw_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->sw_total} : Range{b->s[sl].sw_begin, b->s[sl].sw_n}; }
 
@@ -362,11 +374,15 @@ void iradon_seq(acu_batch* b, const DevArray<RFTask>& rf, const DevArray<BPTask>
   check_launch();
 }
 
-void radon_seq(acu_batch* b, const DevArray<CellTask>& cl, const DevArray<FPTask>& fp, int sl) {
+void radon_seq(acu_batch* b, const DevArray<CellTask>& cl, const DevArray<FPTask>& fp,
+               const DevArray<FPLTask>& fl, int sl) {
   acu_ctx* c = b->ctx;
-  const Range r1 = cl_range(b, sl), r2 = fp_range(b, sl), r3 = fs_range(b, sl);
+  const Range r1 = cl_range(b, sl), r2 = fp_range(b, sl), r3 = fs_range(b, sl), r4 = fl_range(b, sl);
   timed(c, K_CELLS, [&] { launch_cells(cl.d, cl.n, r1.base, r1.n, c->stream); });
-  timed(c, K_FP, [&] { launch_forward(fp.d, fp.n, r2.base, r2.n, r3.base, r3.n, c->stream); });
+  if (b->fp_lds)
+    timed(c, K_FP, [&] { launch_forward_lds(fl.d, fl.n, r4.base, r4.n, b->fpl_al, c->stream); });
+  else
+    timed(c, K_FP, [&] { launch_forward(fp.d, fp.n, r2.base, r2.n, r3.base, r3.n, c->stream); });
   check_launch();
 }
 
@@ -392,14 +408,14 @@ void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool reuse_ruk, bool 
   const int first = sl < 0 ? 0 : sl, count = sl < 0 ? nt : 1;  // finalize: one block per slice
   for (int ji = 0; ji < ninner; ++ji) {
     iradon_seq(b, b->rf_fk, b->bp_ifkt, sl);                       // :215
-    if (!(reuse_ruk && ji == 0)) radon_seq(b, b->cl_uk, b->fp_ruk, sl);  // :228
+    if (!(reuse_ruk && ji == 0)) radon_seq(b, b->cl_uk, b->fp_ruk, b->fl_ruk, sl);  // :228
     iradon_seq(b, b->rf_ruk, b->bp_iukt, sl);                      // :229
     launch_tv_residual(tv, nt, ew.base, ew.n, c->stream);          // :207-249
     launch_tv_finalize(tv, first, count, 0, c->stream);
     check_launch();
     if (host_sync) (void)read_scalar(b, sl, 0);
     for (int j = 0; j < ncg; ++j) {                                // :252-303
-      radon_seq(b, b->cl_p, b->fp_rp, sl);
+      radon_seq(b, b->cl_p, b->fp_rp, b->fl_rp, sl);
       iradon_seq(b, b->rf_rp, b->bp_irpt, sl);
       launch_tv_apdot(tv, nt, ew.base, ew.n, j, c->stream);
       launch_tv_finalize(tv, first, count, 64 + j, c->stream);
@@ -417,7 +433,7 @@ void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool reuse_ruk, bool 
     check_launch();
     if (host_sync) (void)read_scalar(b, sl, 127);
   }
-  radon_seq(b, b->cl_uk, b->fp_ruk, sl);                           // :339
+  radon_seq(b, b->cl_uk, b->fp_ruk, b->fl_ruk, sl);                           // :339
   launch_tv_addback(tv, nt, sw.base, sw.n, c->stream);             // :341-346
   check_launch();
 }
@@ -449,7 +465,7 @@ void tvopt_run(acu_batch* b, float mu, float lambda, int nouter, int ninner, int
     for (int sl = 0; sl < (int)b->s.size(); ++sl)
       for (int jo = 0; jo < nouter; ++jo) outer_body(b, sl, ninner, ncg, false, true);
   } else {
-    if (nouter > 0) radon_seq(b, b->cl_uk, b->fp_ruk, -1);  // ruk = radon(uk) for :228
+    if (nouter > 0) radon_seq(b, b->cl_uk, b->fp_ruk, b->fl_ruk, -1);  // ruk = radon(uk) for :228
     const bool graph = (mode == ACU_TV_GRAPH) && !c->timing;
     if (!graph) {
       for (int jo = 0; jo < nouter; ++jo) outer_body(b, -1, ninner, ncg, true, false);
@@ -505,7 +521,12 @@ void radon_dev(acu_batch* b, int sl, float* sino, const float* img) {
     CellTask* d = put_adhoc(b, t);
     timed(c, K_CELLS, [&] { launch_cells(d, 1, 0, s.cl_n, c->stream); });
   }
-  {
+  if (b->fp_lds) {
+    FPLTask t = fpl_task(s, img, sino);
+    t.block_begin = 0;
+    FPLTask* d = put_adhoc(b, t);
+    timed(c, K_FP, [&] { launch_forward_lds(d, 1, 0, s.fl_n, b->fpl_al, c->stream); });
+  } else {
     FPTask t = fp_task(s, sino);
     t.block_begin = 0;
     t.sum_begin = 0;
@@ -615,7 +636,10 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
   try {
     ACU_HIP_CHECK(hipSetDevice(ctx->device));
     b->s.resize(nslices);
-    int rf = 0, bp = 0, cl = 0, fp = 0, fs = 0, ew = 0, sw = 0;
+    int rf = 0, bp = 0, cl = 0, fp = 0, fs = 0, fl = 0, ew = 0, sw = 0;
+    // tuning: forward-projection path, ACUMEN_FP_PATH = lds (default) | gather
+    const char* pv = std::getenv("ACUMEN_FP_PATH");
+    b->fp_lds = !(pv && std::strcmp(pv, "gather") == 0);
     // tuning: forward-projection sweep chunk length (steps), ACUMEN_FP_CHUNK
     const char* ev = std::getenv("ACUMEN_FP_CHUNK");
     const int fp_clen_env = ev ? std::atoi(ev) : 0;
@@ -631,10 +655,17 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       s.fp_clen = fp_clen_env > 0 ? fp_clen_env : acu::FP_CHUNK_DEFAULT;
       s.fp_begin = fp; s.fp_n = acu::fp_ray_blocks(s.N) * s.T * acu::fp_chunks(s.N, s.fp_clen); fp += s.fp_n;
       s.fs_begin = fs; s.fs_n = acu::fp_sum_blocks(s.N, s.T); fs += s.fs_n;
+      b->fpl_al = std::min(b->fpl_al, acu::fpl_angles_per_lane(s.N, s.T));
       s.ew_begin = ew; s.ew_n = acu::ew_blocks((size_t)s.N * s.N); ew += s.ew_n;
       s.sw_begin = sw; s.sw_n = acu::ew_blocks((size_t)s.N * s.T); sw += s.sw_n;
     }
+    // one angle-block width for the whole batch (the kernel is specialised on it)
+    for (auto& s : b->s) {
+      s.fpl_al = b->fpl_al;
+      s.fl_begin = fl; s.fl_n = acu::fpl_blocks(s.N, s.T, s.fpl_al); fl += s.fl_n;
+    }
     b->rf_total = rf; b->bp_total = bp; b->cl_total = cl; b->fp_total = fp; b->fs_total = fs;
+    b->fl_total = fl;
     b->ew_total = ew; b->sw_total = sw;
     acu::Carver dry;
     acu::layout_ops(b.get(), dry);

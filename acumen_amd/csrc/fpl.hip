@@ -1,0 +1,291 @@
+// fpl.hip -- forward projection (radon, tomography/tv_hpc.F90:129-159) with
+// the image staged in LDS and shared by a block of neighbouring angles.
+//
+// Same definition as fp.hip:
+//   sino(y,t) = (1/N) * sum_{x=1..N} bilinear(img; xr, yr), x in order,
+//   xr = cos(c)*(x-m) - sin(c)*(y-m) + m,  yr = sin(c)*(x-m) + cos(c)*(y-m) + m,
+//   a sample counts only if its floor cell (x1, y1) has 2 <= x1, y1 <= N-1.
+//
+// gfx950 mapping.  One workgroup = 64 neighbouring rays (one lane each) x a
+// block of 4*al neighbouring angles (al per lane, one wave per al angles).
+// Every lane sums its rays' samples in step order, so the per-ray summation
+// order is the reference's.  The sweep goes in chunks of FPL_U steps; for a
+// chunk the workgroup stages in LDS the bounding box of all its samples
+// (raw FP32 pixels, zero outside the image, fixed row pitch FPL_PITCH so
+// both rows of a bilinear cell come from ds_read2 immediates) and every
+// angle of the block
+// reads it, so one staged pixel serves ~2-3 samples and the gathers never
+// leave the CU (the global-gather kernel of fp.hip moves ~23 GB per
+// 2049^2 x 1500 launch; this one stages ~1.7 bytes per sample from L2).
+//
+//  * interior chunks (every cell of the box passes the bounds test):
+//    positions are FMAs from the ray's start, there is no test, and a sample
+//    is 2 FMA + 2 cvt + 2 fract + addressing + 2 ds_read2_b32 + 3 lerps;
+//  * border chunks: positions are the reference's FP32 values (c*u, s*u,
+//    s*ym, c*ym rounded once, summed in its order, no contraction) and the
+//    bounds test is applied, so every edge decision is the reference's own;
+//  * a chunk whose box does not fit the LDS budget falls back to gathering
+//    the bilinear cells of fp.hip from global memory (exact positions).
+#include "acu_internal.hpp"
+
+namespace acu {
+namespace {
+
+constexpr int FPL_THREADS = 256;
+constexpr int AMAX = 4;  // max angles per lane
+constexpr int FPL_SB = 8;  // staging loads in flight per lane
+
+__device__ __forceinline__ int find_task(const FPLTask* tasks, int ntasks, int gblock) {
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].block_begin <= gblock) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ float wmin(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ float wmax(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Range of step offsets u = x - m over which a + b*u stays in [lo, hi].
+__device__ __forceinline__ void urange(float a, float b, float lo, float hi, float& u0, float& u1) {
+  if (fabsf(b) < 1e-6f) {
+    if (a >= lo && a <= hi) { u0 = -1e30f; u1 = 1e30f; } else { u0 = 1e30f; u1 = -1e30f; }
+  } else {
+    const float p = (lo - a) / b, q = (hi - a) / b;
+    u0 = fminf(p, q);
+    u1 = fmaxf(p, q);
+  }
+}
+
+template <int AL>
+__global__ __launch_bounds__(FPL_THREADS) void fpl_kernel(const FPLTask* __restrict__ tasks, int ntasks,
+                                                          int base, int nblocks) {
+#pragma clang fp contract(off)
+  constexpr int PITCH = FPL_PITCH;
+  __shared__ float box[FPL_CAP];
+  __shared__ float red[4][4];  // per-wave min x, min y, max x, max y (or step range)
+
+  const int lbk = xcd_remap(blockIdx.x, gridDim.x);
+  if (lbk >= nblocks) return;
+  const int gb = base + lbk;
+  const FPLTask& tk = tasks[find_task(tasks, ntasks, gb)];
+  const int N = tk.N, T = tk.T;
+  const float m = tk.mid;
+  const int local = gb - tk.block_begin;
+  const int rb = local / tk.nab, ab = local - rb * tk.nab;  // ray-block-major
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ray = rb * 64 + lane;
+  const float ym = (float)(min(ray, N - 1) + 1) - m;
+
+  float s[AL], c[AL], sy[AL], cyv[AL], acc[AL];
+  bool act[AL];
+  int ang[AL];
+  float lu0 = 1e30f, lu1 = -1e30f;
+#pragma unroll
+  for (int j = 0; j < AL; ++j) {
+    ang[j] = ab * 4 * AL + wave * AL + j;
+    act[j] = ang[j] < T && ray < N;
+    const float2 v = const_f2(tk.sc, min(ang[j], T - 1));
+    s[j] = v.x;
+    c[j] = v.y;
+    sy[j] = s[j] * ym;   // one rounding each (tv_hpc.F90:143-144)
+    cyv[j] = c[j] * ym;
+    acc[j] = 0.0f;
+    if (act[j]) {
+      float ua, ub, va, vb;
+      urange(m - sy[j], c[j], 1.99f, (float)N + 0.01f, ua, ub);
+      urange(m + cyv[j], s[j], 1.99f, (float)N + 0.01f, va, vb);
+      const float u0 = fmaxf(ua, va), u1 = fminf(ub, vb);
+      if (u0 <= u1) { lu0 = fminf(lu0, u0); lu1 = fmaxf(lu1, u1); }
+    }
+  }
+  // workgroup step window (1-based steps, one step of margin)
+  lu0 = wmin(lu0);
+  lu1 = wmax(lu1);
+  if (lane == 0) { red[wave][0] = lu0; red[wave][1] = lu1; }
+  __syncthreads();
+  const float U0 = fminf(fminf(red[0][0], red[1][0]), fminf(red[2][0], red[3][0]));
+  const float U1 = fmaxf(fmaxf(red[0][1], red[1][1]), fmaxf(red[2][1], red[3][1]));
+  __syncthreads();
+  int XS = 1, XE = 0;
+  if (U0 <= U1) {
+    XS = max(1, (int)floorf(fmaxf(U0 + m, -1e9f)) - 1);
+    XE = min(N, (int)ceilf(fminf(U1 + m, 1e9f)) + 1);
+  }
+
+  const ACU_GLOBAL float* __restrict__ img = as_global(tk.img);
+  for (int c0 = XS; c0 <= XE; c0 += FPL_U) {
+    const int c1 = min(XE, c0 + FPL_U - 1);
+    // bounding box of the chunk's sample positions (fast form, affine in the
+    // step: extremes at c0 and c1)
+    float mnx = 1e30f, mny = 1e30f, mxx = -1e30f, mxy = -1e30f;
+    const float ua = (float)c0 - m, ub = (float)c1 - m;
+#pragma unroll
+    for (int j = 0; j < AL; ++j) {
+      if (!act[j]) continue;
+      const float xa = fmaf(c[j], ua, m - sy[j]), xb = fmaf(c[j], ub, m - sy[j]);
+      const float ya = fmaf(s[j], ua, m + cyv[j]), yb = fmaf(s[j], ub, m + cyv[j]);
+      mnx = fminf(mnx, fminf(xa, xb)); mxx = fmaxf(mxx, fmaxf(xa, xb));
+      mny = fminf(mny, fminf(ya, yb)); mxy = fmaxf(mxy, fmaxf(ya, yb));
+    }
+    mnx = wmin(mnx); mny = wmin(mny); mxx = wmax(mxx); mxy = wmax(mxy);
+    if (lane == 0) { red[wave][0] = mnx; red[wave][1] = mny; red[wave][2] = mxx; red[wave][3] = mxy; }
+    __syncthreads();
+    mnx = fminf(fminf(red[0][0], red[1][0]), fminf(red[2][0], red[3][0]));
+    mny = fminf(fminf(red[0][1], red[1][1]), fminf(red[2][1], red[3][1]));
+    mxx = fmaxf(fmaxf(red[0][2], red[1][2]), fmaxf(red[2][2], red[3][2]));
+    mxy = fmaxf(fmaxf(red[0][3], red[1][3]), fmaxf(red[2][3], red[3][3]));
+    __syncthreads();
+    if (!(mnx <= mxx)) continue;  // no active lane in this workgroup (uniform)
+    // 1-based pixel box with two pixels of margin on every side
+    const int bx0 = (int)floorf(fmaxf(mnx, -1e6f)) - 2, by0 = (int)floorf(fmaxf(mny, -1e6f)) - 2;
+    const int bw = (int)floorf(fminf(mxx, 1e6f)) - bx0 + 4, bh = (int)floorf(fminf(mxy, 1e6f)) - by0 + 4;
+    const bool fits = bw > 0 && bh > 0 && bw <= PITCH && bh * PITCH <= FPL_CAP;
+    if (fits) {
+      // flat staging, FPL_SB independent loads in flight per lane; the row
+      // index (i + 0.5) / bw in FP32 is exact: its fraction stays >= 0.5/bw
+      // from an integer while the rounding error is below 2e-3
+      const int area = bw * bh;
+      const float rbw = 1.0f / (float)bw;
+      for (int i0 = threadIdx.x; i0 < area; i0 += FPL_THREADS * FPL_SB) {
+        float v[FPL_SB];
+        int o[FPL_SB];
+#pragma unroll
+        for (int k = 0; k < FPL_SB; ++k) {
+          const int i = i0 + k * FPL_THREADS;
+          const int ly = (int)(((float)i + 0.5f) * rbw);
+          const int lx = i - __mul24(ly, bw);
+          const int gx = bx0 + lx, gy = by0 + ly;
+          o[k] = i < area ? __mul24(ly, PITCH) + lx : -1;
+          v[k] = 0.0f;
+          if (i < area && gx >= 1 && gx <= N && gy >= 1 && gy <= N)
+            v[k] = img[(size_t)(gy - 1) * N + (gx - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < FPL_SB; ++k)
+          if (o[k] >= 0) box[o[k]] = v[k];
+      }
+      __syncthreads();
+      const bool interior = bx0 >= 2 && by0 >= 2 && bx0 + bw - 1 <= N && by0 + bh - 1 <= N;
+      if (interior) {
+        // every cell in the box passes the bounds test: fast positions, no
+        // branches; an inactive (ray, angle) reads a fixed in-box pixel and
+        // its sum is never stored
+        float bxl[AL], byl[AL], cj[AL], sj[AL];
+#pragma unroll
+        for (int j = 0; j < AL; ++j) {
+          cj[j] = act[j] ? c[j] : 0.0f;
+          sj[j] = act[j] ? s[j] : 0.0f;
+          bxl[j] = act[j] ? (m - sy[j]) - (float)bx0 : 1.0f;
+          byl[j] = act[j] ? (m + cyv[j]) - (float)by0 : 1.0f;
+        }
+#pragma unroll 2
+        for (int x = c0; x <= c1; ++x) {
+          const float u = (float)x - m;
+          float val[AL];
+#pragma unroll
+          for (int j = 0; j < AL; ++j) {
+            const float xl = fmaf(cj[j], u, bxl[j]);
+            const float yl = fmaf(sj[j], u, byl[j]);
+            const int ix = (int)xl, iy = (int)yl;
+            const float fx = __builtin_amdgcn_fractf(xl), fy = __builtin_amdgcn_fractf(yl);
+            const float* p0 = box + (__mul24(iy, PITCH) + ix);
+            const float v00 = p0[0], v10 = p0[1], v01 = p0[PITCH], v11 = p0[PITCH + 1];
+            const float top = fmaf(fx, v10 - v00, v00);
+            const float bot = fmaf(fx, v11 - v01, v01);
+            val[j] = fmaf(fy, bot - top, top);
+          }
+#pragma unroll
+          for (int j = 0; j < AL; ++j) acc[j] += val[j];
+        }
+      } else {
+        // border chunk: the reference's positions and bounds test
+        for (int x = c0; x <= c1; ++x) {
+          const float u = (float)x - m;
+#pragma unroll
+          for (int j = 0; j < AL; ++j) {
+            if (!act[j]) continue;
+            const float xr = (c[j] * u - sy[j]) + m;
+            const float yr = (s[j] * u + cyv[j]) + m;
+            const float fxr = floorf(xr), fyr = floorf(yr);
+            const int x1 = (int)fxr, y1 = (int)fyr;
+            if (x1 >= 2 && x1 <= N - 1 && y1 >= 2 && y1 <= N - 1) {
+              const float fx = xr - fxr, fy = yr - fyr;
+              const float* p0 = box + (__mul24(y1 - by0, PITCH) + (x1 - bx0));
+              const float v00 = p0[0], v10 = p0[1], v01 = p0[PITCH], v11 = p0[PITCH + 1];
+              const float top = fmaf(fx, v10 - v00, v00);
+              const float bot = fmaf(fx, v11 - v01, v01);
+              acc[j] += fmaf(fy, bot - top, top);
+            }
+          }
+        }
+      }
+      __syncthreads();
+    } else {
+      // box too large for LDS: bilinear cells from global memory
+      const int P = N + 1;
+      const float fN = (float)N;
+      for (int x = c0; x <= c1; ++x) {
+        const float u = (float)x - m;
+#pragma unroll
+        for (int j = 0; j < AL; ++j) {
+          if (!act[j]) continue;
+          const float xr = (c[j] * u - sy[j]) + m;
+          const float yr = (s[j] * u + cyv[j]) + m;
+          const int cx = (int)__builtin_amdgcn_fmed3f(xr, 0.0f, fN);
+          const int cyi = (int)__builtin_amdgcn_fmed3f(yr, 0.0f, fN);
+          const float wx = __builtin_amdgcn_fractf(xr), wy = __builtin_amdgcn_fractf(yr);
+          const float4 q = global_f4(as_global(tk.cells), (unsigned)(__mul24(cyi, P) + cx));
+          const float t1 = fmaf(wx, q.w, q.z);
+          const float t2 = fmaf(wx, q.y, q.x);
+          acc[j] += fmaf(wy, t1, t2);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < AL; ++j)
+    if (act[j]) tk.sino[(size_t)ang[j] * N + ray] = acc[j] / (float)N;
+}
+
+}  // namespace
+
+int fpl_angles_per_lane(int N, int T) {
+  // keep the spread of a 4*al angle block (radius ~0.75 * N/sqrt 2) within
+  // what the LDS box leaves after the 64-ray x FPL_U-step footprint;
+  // al in {1, 2, 4}
+  const double dth = 3.141592653589793 / (double)T;
+  const double r = 0.75 * 0.7071 * (double)N;
+  const double room = (double)FPL_PITCH - 0.7071 * (64 + FPL_U) - 6.0;
+  int al = 1;
+  while (al < AMAX && (4.0 * (2 * al) - 1.0) * dth * r <= room) al *= 2;
+  return al;
+}
+int fpl_blocks(int N, int T, int al) {
+  const int nrb = (N + 63) / 64;
+  const int nab = (T + 4 * al - 1) / (4 * al);
+  return nrb * nab;
+}
+
+void launch_forward_lds(const FPLTask* d_tasks, int ntasks, int base, int blocks, int al,
+                        hipStream_t s) {
+  if (blocks <= 0) return;
+  const dim3 g(xcd_grid(blocks)), t(FPL_THREADS);
+  if (al >= 4)
+    hipLaunchKernelGGL(fpl_kernel<4>, g, t, 0, s, d_tasks, ntasks, base, blocks);
+  else if (al == 2)
+    hipLaunchKernelGGL(fpl_kernel<2>, g, t, 0, s, d_tasks, ntasks, base, blocks);
+  else
+    hipLaunchKernelGGL(fpl_kernel<1>, g, t, 0, s, d_tasks, ntasks, base, blocks);
+}
+
+}  // namespace acu
