@@ -68,9 +68,19 @@ constexpr int FP_THREADS = 256;  // rays per workgroup (one angle per workgroup)
 constexpr int FP_CHUNK_DEFAULT = 1 << 20;  // steps per sweep chunk (default: the whole ray)
 
 // ---- forward projection with LDS-staged boxes (fpl.hip) ----
-constexpr int FPL_U = 32;          // steps per staged chunk
-constexpr int FPL_PITCH = 111;     // LDS row pitch of the box (odd; pitch+1 <= 255 dwords)
-constexpr int FPL_CAP = 111 * 110; // floats of the LDS box (~48 KB): 111 x 110 pixels
+#ifndef ACUMEN_FPL_U
+#define ACUMEN_FPL_U 32
+#endif
+#ifndef ACUMEN_FPL_PITCH
+#define ACUMEN_FPL_PITCH 101
+#endif
+#ifndef ACUMEN_FPL_ROWS
+#define ACUMEN_FPL_ROWS 101
+#endif
+constexpr int FPL_U = ACUMEN_FPL_U;          // steps per staged chunk
+constexpr int FPL_PITCH = ACUMEN_FPL_PITCH;  // LDS row pitch of the box (odd; pitch+1 <= 255 dwords)
+constexpr int FPL_CAP = ACUMEN_FPL_PITCH * ACUMEN_FPL_ROWS;  // floats of the LDS box
+static_assert(FPL_PITCH + 1 <= 255, "ds_read2 offsets");
 
 // ---- ramp filter GEMM (filter.hip) ----
 constexpr int RF_BM = 64;   // detector rows per tile

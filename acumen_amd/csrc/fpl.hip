@@ -32,7 +32,10 @@ namespace acu {
 namespace {
 
 constexpr int FPL_THREADS = 256;
-constexpr int AMAX = 4;  // max angles per lane
+#ifndef ACUMEN_FPL_AMAX
+#define ACUMEN_FPL_AMAX 2
+#endif
+constexpr int AMAX = ACUMEN_FPL_AMAX;  // max angles per lane (1, 2 or 4)
 constexpr int FPL_SB = 8;  // staging loads in flight per lane
 
 __device__ __forceinline__ int find_task(const FPLTask* tasks, int ntasks, int gblock) {
@@ -151,28 +154,34 @@ __global__ __launch_bounds__(FPL_THREADS) void fpl_kernel(const FPLTask* __restr
     const int bw = (int)floorf(fminf(mxx, 1e6f)) - bx0 + 4, bh = (int)floorf(fminf(mxy, 1e6f)) - by0 + 4;
     const bool fits = bw > 0 && bh > 0 && bw <= PITCH && bh * PITCH <= FPL_CAP;
     if (fits) {
-      // flat staging, FPL_SB independent loads in flight per lane; the row
-      // index (i + 0.5) / bw in FP32 is exact: its fraction stays >= 0.5/bw
-      // from an integer while the rounding error is below 2e-3
-      const int area = bw * bh;
-      const float rbw = 1.0f / (float)bw;
-      for (int i0 = threadIdx.x; i0 < area; i0 += FPL_THREADS * FPL_SB) {
-        float v[FPL_SB];
-        int o[FPL_SB];
+      // row staging: lanes cover 128 columns, the wave pair (wave >> 1) picks
+      // the row parity, FPL_SB loads in flight per lane; 32-bit byte offsets
+      // from the image base and LDS stores at immediate offsets
+      {
+        const int rp = __builtin_amdgcn_readfirstlane(wave >> 1);
+        const int col = lane + ((wave & 1) << 6);
+        const int gx = bx0 + col;
+        const bool cok = col < bw;
+        const bool xin = cok && gx >= 1 && gx <= N;
+        const ACU_GLOBAL char* ib = (const ACU_GLOBAL char*)img;
+        unsigned goff = 4u * (unsigned)((by0 + rp - 1) * N + (gx - 1));  // wraps for rows above the image
+        const unsigned gstep = 8u * (unsigned)N;
+        float* lrow = box + (rp * PITCH + col);
+        for (int ly0 = rp; ly0 < bh; ly0 += 2 * FPL_SB) {
+          float v[FPL_SB];
 #pragma unroll
-        for (int k = 0; k < FPL_SB; ++k) {
-          const int i = i0 + k * FPL_THREADS;
-          const int ly = (int)(((float)i + 0.5f) * rbw);
-          const int lx = i - __mul24(ly, bw);
-          const int gx = bx0 + lx, gy = by0 + ly;
-          o[k] = i < area ? __mul24(ly, PITCH) + lx : -1;
-          v[k] = 0.0f;
-          if (i < area && gx >= 1 && gx <= N && gy >= 1 && gy <= N)
-            v[k] = img[(size_t)(gy - 1) * N + (gx - 1)];
+          for (int k = 0; k < FPL_SB; ++k) {
+            const int gy = by0 + ly0 + 2 * k;
+            v[k] = 0.0f;
+            if (xin && ly0 + 2 * k < bh && gy >= 1 && gy <= N)
+              v[k] = *(const ACU_GLOBAL float*)(ib + (goff + (unsigned)k * gstep));
+          }
+#pragma unroll
+          for (int k = 0; k < FPL_SB; ++k)
+            if (cok && ly0 + 2 * k < bh) lrow[2 * k * PITCH] = v[k];
+          goff += (unsigned)FPL_SB * gstep;
+          lrow += 2 * FPL_SB * PITCH;
         }
-#pragma unroll
-        for (int k = 0; k < FPL_SB; ++k)
-          if (o[k] >= 0) box[o[k]] = v[k];
       }
       __syncthreads();
       const bool interior = bx0 >= 2 && by0 >= 2 && bx0 + bw - 1 <= N && by0 + bh - 1 <= N;
