@@ -24,7 +24,7 @@ ACU_TV_PLAIN = 0
 ACU_TV_BATCHED = 1
 ACU_TV_GRAPH = 2
 
-KIND_RAMP, KIND_BP, KIND_CELLS, KIND_FP = 0, 1, 2, 3
+KIND_RAMP, KIND_BP, KIND_CELLS, KIND_FP, KIND_ADJ = 0, 1, 2, 3, 4
 
 # Every symbol the header declares, with (restype, argtypes).
 _c_int, _c_float, _vp = ctypes.c_int, ctypes.c_float, ctypes.c_void_p
@@ -55,6 +55,14 @@ SIGNATURES = {
     "acu_iradon": (_c_int, [_vp, _fp, _fp]),
     "acu_get_slice": (_c_int, [_vp, _fp, _fp]),
     "acu_tvopt": (_c_int, [_vp, _fp, _c_float, _c_float, _c_int, _c_int, _c_int, _fp, _fp, _fp]),
+    "acu_radon_adjoint_dev": (_c_int, [_vp, _c_int, _vp, _vp]),
+    "acu_radon_adjoint": (_c_int, [_vp, _fp, _fp]),
+    "acu_batch_loss_grad": (_c_int, [_vp, ctypes.POINTER(_fp), ctypes.POINTER(_fp), _c_float, _c_float,
+                                     ctypes.POINTER(_fp), ctypes.POINTER(_fp), ctypes.POINTER(_fp),
+                                     ctypes.POINTER(ctypes.c_double)]),
+    "acu_batch_loss_grad_dev": (_c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _c_float, _c_float,
+                                         ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                         ctypes.POINTER(ctypes.c_double)]),
     "acu_ctx_set_timing": (_c_int, [_vp, _c_int]),
     "acu_ctx_kernel_time": (_c_int, [_vp, _c_int, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_longlong)]),

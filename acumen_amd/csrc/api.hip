@@ -2,6 +2,7 @@
 // reconstruction library: plans/batches (the reference's module state),
 // operator launches, and the split-Bregman TV orchestration.
 #include "acu_tv.hpp"
+#include "acu_grad.hpp"
 #include "acumen.h"
 
 #include <algorithm>
@@ -28,8 +29,9 @@ struct acu_ctx {
   struct Pending { int kind; hipEvent_t a, b; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
-  double ms[4] = {0, 0, 0, 0};
-  long long count[4] = {0, 0, 0, 0};
+  static constexpr int KINDS = 5;  // ramp, backprojection, cells, forward, adjoint
+  double ms[KINDS] = {0, 0, 0, 0, 0};
+  long long count[KINDS] = {0, 0, 0, 0, 0};
 
   hipEvent_t ev() {
     if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
@@ -156,6 +158,12 @@ struct Slice {
   float* delta = nullptr;
   int delta_cap = 0;
   bool fbp_valid = false, tv_valid = false;
+  // differentiation arena (allocated on first use): image, its gradient,
+  // radon(x), shifted residual back-shifted, shifts, shift gradient,
+  // data / TV partials and the three loss values
+  float *gx_x = nullptr, *gx_g = nullptr, *gx_y = nullptr, *gx_rs = nullptr, *gx_sh = nullptr,
+        *gx_gs = nullptr, *gx_hs = nullptr;
+  double *gx_pd = nullptr, *gx_pt = nullptr, *gx_loss = nullptr;
   // global block ranges in the batched launches
   int rf_begin = 0, rf_n = 0, bp_begin = 0, bp_n = 0, cl_begin = 0, cl_n = 0,
       fp_begin = 0, fp_n = 0, fs_begin = 0, fs_n = 0, fl_begin = 0, fl_n = 0, ew_begin = 0, ew_n = 0, sw_begin = 0, sw_n = 0;
@@ -184,6 +192,15 @@ struct acu_batch {
   float tv_mu = 0.f, tv_lambda = 0.f;
   // single-slot task buffer for calls on caller-provided device pointers
   void* adhoc = nullptr;
+  // differentiation: arena and per-call task sets
+  void* grad_mem = nullptr;
+  acu::DevArray<acu::CellTask> cl_g;
+  acu::DevArray<acu::FPLTask> fl_g;
+  acu::DevArray<acu::FPTask> fp_g;
+  acu::DevArray<acu::ResTask> res_g;
+  acu::DevArray<acu::AdjTask> adj_g;
+  acu::DevArray<acu::TVGTask> tvg_g;
+  acu::DevArray<acu::LossTask> loss_g;
   // graph of one outer iteration
   hipGraphExec_t graph = nullptr;
   int graph_ninner = -1, graph_ncg = -1;
@@ -194,6 +211,9 @@ struct acu_batch {
     bp_fbp.release(); bp_ifkt.release(); bp_iukt.release(); bp_irpt.release();
     cl_uk.release(); cl_p.release(); fp_ruk.release(); fp_rp.release(); tv.release();
     fl_ruk.release(); fl_rp.release();
+    cl_g.release(); fl_g.release(); fp_g.release(); res_g.release(); adj_g.release();
+    tvg_g.release(); loss_g.release();
+    if (grad_mem) hipFree(grad_mem);
     if (adhoc) hipFree(adhoc);
     if (tv_mem) hipFree(tv_mem);
     if (ops_mem) hipFree(ops_mem);
@@ -203,7 +223,7 @@ struct acu_batch {
 namespace acu {
 namespace {
 
-enum Kind { K_RF = 0, K_BP = 1, K_CELLS = 2, K_FP = 3 };
+enum Kind { K_RF = 0, K_BP = 1, K_CELLS = 2, K_FP = 3, K_ADJ = 4 };
 
 template <class F>
 void timed(acu_ctx* c, int kind, F&& launch) {
@@ -556,6 +576,118 @@ void bp_dev(acu_batch* b, int sl, float* img, const float* filt) {
   check_launch();
 }
 
+// ---- differentiation: exact adjoint and the gradients of
+// L(x, s) = 1/2 |shift_s(radon(x)) - b|^2 + lambda TV_eps(x), b = the slice's f ----
+AdjTask adj_task(const Slice& s, int block_begin, const float* sino, float* out) {
+  return AdjTask{s.N, s.T, s.mid, block_begin, adj_tiles(s.N), sino, s.scfp, out};
+}
+
+void radon_adjoint_dev(acu_batch* b, int sl, float* img, const float* sino) {
+  Slice& s = slice_of(b, sl);
+  acu_ctx* c = b->ctx;
+  AdjTask* d = put_adhoc(b, adj_task(s, 0, sino, img));
+  const int nb = adj_tiles(s.N) * adj_tiles(s.N);
+  timed(c, K_ADJ, [&] { launch_adjoint(d, 1, 0, nb, c->stream); });
+  check_launch();
+}
+
+void layout_grad(acu_batch* b, Carver& cv) {
+  for (auto& s : b->s) {
+    const size_t NN = (size_t)s.N * s.N, NT = (size_t)s.N * s.T;
+    s.gx_x = cv.take<float>(NN);
+    s.gx_g = cv.take<float>(NN);
+    s.gx_y = cv.take<float>(NT);
+    s.gx_rs = cv.take<float>(NT);
+    s.gx_sh = cv.take<float>((size_t)s.T);
+    s.gx_gs = cv.take<float>((size_t)s.T);
+    s.gx_hs = cv.take<float>((size_t)s.T);
+    s.gx_pd = cv.take<double>((size_t)s.T);
+    s.gx_pt = cv.take<double>((size_t)tvg_blocks(s.N));
+    s.gx_loss = cv.take<double>(4);
+  }
+}
+
+void ensure_grad(acu_batch* b) {
+  if (b->grad_mem) return;
+  Carver dry;
+  layout_grad(b, dry);
+  ACU_HIP_CHECK(hipMalloc(&b->grad_mem, dry.off + 256));
+  Carver cv{static_cast<char*>(b->grad_mem), 0};
+  layout_grad(b, cv);
+}
+
+// radon of every slice: imgs[i] -> sinos[i] (the batch's block ranges)
+void radon_many(acu_batch* b, const std::vector<const float*>& imgs, const std::vector<float*>& sinos) {
+  acu_ctx* c = b->ctx;
+  const int n = (int)b->s.size();
+  std::vector<CellTask> cl(n);
+  for (int i = 0; i < n; ++i) cl[i] = cell_task(b->s[i], imgs[i]);
+  b->cl_g.upload(cl);
+  timed(c, K_CELLS, [&] { launch_cells(b->cl_g.d, n, 0, b->cl_total, c->stream); });
+  if (b->fp_lds) {
+    std::vector<FPLTask> fl(n);
+    for (int i = 0; i < n; ++i) fl[i] = fpl_task(b->s[i], imgs[i], sinos[i]);
+    b->fl_g.upload(fl);
+    timed(c, K_FP, [&] { launch_forward_lds(b->fl_g.d, n, 0, b->fl_total, b->fpl_al, c->stream); });
+  } else {
+    std::vector<FPTask> fp(n);
+    for (int i = 0; i < n; ++i) fp[i] = fp_task(b->s[i], sinos[i]);
+    b->fp_g.upload(fp);
+    timed(c, K_FP, [&] { launch_forward(b->fp_g.d, n, 0, b->fp_total, 0, b->fs_total, c->stream); });
+  }
+  check_launch();
+}
+
+// x[i], shift[i] (nullable), gx[i], gs[i], hs[i] (nullable) are device
+// pointers; gx may be empty (no image gradient).  loss receives 3 doubles
+// per slice.
+void loss_grad_run(acu_batch* b, const std::vector<const float*>& x, const std::vector<const float*>& shift,
+                   float lambda, float eps, const std::vector<float*>& gx, const std::vector<float*>& gs,
+                   const std::vector<float*>& hs, double* loss) {
+  acu_ctx* c = b->ctx;
+  require_theta(b);
+  ensure_grad(b);
+  const int n = (int)b->s.size();
+  std::vector<float*> ys(n);
+  for (int i = 0; i < n; ++i) ys[i] = b->s[i].gx_y;
+  radon_many(b, x, ys);
+  std::vector<ResTask> res(n);
+  std::vector<TVGTask> tvg(n);
+  std::vector<LossTask> lt(n);
+  std::vector<AdjTask> adj;
+  int rows = 0, tvb = 0, adjb = 0;
+  for (int i = 0; i < n; ++i) {
+    Slice& s = b->s[i];
+    res[i] = ResTask{s.N, s.T, rows, s.gx_y, s.f, shift[i], s.gx_rs, gs[i], hs[i], s.gx_pd};
+    rows += s.T;
+    const int nb = tvg_blocks(s.N);
+    tvg[i] = TVGTask{s.N, tvb, nb, lambda, eps, x[i], gx.empty() ? nullptr : gx[i], s.gx_pt};
+    tvb += nb;
+    lt[i] = LossTask{s.T, nb, (double)lambda, s.gx_pd, s.gx_pt, s.gx_loss};
+    if (!gx.empty()) {
+      adj.push_back(adj_task(s, adjb, s.gx_rs, gx[i]));
+      adjb += adj_tiles(s.N) * adj_tiles(s.N);
+    }
+  }
+  b->res_g.upload(res);
+  b->tvg_g.upload(tvg);
+  b->loss_g.upload(lt);
+  launch_shift_resid(b->res_g.d, n, 0, rows, c->stream);
+  check_launch();
+  if (!adj.empty()) {
+    b->adj_g.upload(adj);
+    timed(c, K_ADJ, [&] { launch_adjoint(b->adj_g.d, n, 0, adjb, c->stream); });
+    check_launch();
+  }
+  launch_tvgrad(b->tvg_g.d, n, 0, tvb, c->stream);
+  launch_loss_finalize(b->loss_g.d, n, c->stream);
+  check_launch();
+  for (int i = 0; i < n; ++i)
+    ACU_HIP_CHECK(hipMemcpyAsync(loss + 3 * i, b->s[i].gx_loss, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+  c->sync();
+}
+
 }  // namespace
 }  // namespace acu
 
@@ -612,11 +744,11 @@ int acu_ctx_synchronize(acu_ctx* ctx) {
 int acu_ctx_set_timing(acu_ctx* ctx, int enable) {
   if (!ctx) return fail(ACU_ERR_ARG, "null context");
   ACU_TRY(ctx->sync(); ctx->timing = enable != 0;
-          for (int k = 0; k < 4; ++k) { ctx->ms[k] = 0; ctx->count[k] = 0; });
+          for (int k = 0; k < acu_ctx::KINDS; ++k) { ctx->ms[k] = 0; ctx->count[k] = 0; });
 }
 
 int acu_ctx_kernel_time(acu_ctx* ctx, int kind, double* total_ms, long long* launches) {
-  if (!ctx || kind < 0 || kind > 3) return fail(ACU_ERR_ARG, "acu_ctx_kernel_time: bad argument");
+  if (!ctx || kind < 0 || kind >= acu_ctx::KINDS) return fail(ACU_ERR_ARG, "acu_ctx_kernel_time: bad argument");
   ACU_TRY(ctx->sync(); if (total_ms) *total_ms = ctx->ms[kind];
           if (launches) *launches = ctx->count[kind]);
 }
@@ -840,6 +972,96 @@ int acu_get_slice(acu_batch* b, const float* f_host, float* img_fbp_host) {
   if (rc) return rc;
   if ((rc = acu_batch_fbp(b))) return rc;
   return acu_batch_get_image(b, 0, ACU_IMG_FBP, img_fbp_host);
+}
+
+int acu_radon_adjoint_dev(acu_batch* b, int slice, float* img_dev, const float* sino_dev) {
+  if (!b || !img_dev || !sino_dev) return fail(ACU_ERR_ARG, "acu_radon_adjoint_dev: null argument");
+  ACU_TRY(hipSetDevice(b->ctx->device); acu::radon_adjoint_dev(b, slice, img_dev, sino_dev));
+}
+
+int acu_radon_adjoint(acu_batch* b, float* img_host, const float* sino_host) {
+  if (!b || !img_host || !sino_host) return fail(ACU_ERR_ARG, "acu_radon_adjoint: null argument");
+  ACU_TRY(hipSetDevice(b->ctx->device);
+          acu::Slice& s = acu::slice_of(b, 0);
+          DevBuf img((size_t)s.N * s.N), sino((size_t)s.N * s.T);
+          ACU_HIP_CHECK(hipMemcpy(sino.p, sino_host, (size_t)s.N * s.T * 4, hipMemcpyHostToDevice));
+          acu::radon_adjoint_dev(b, 0, img.p, sino.p);
+          b->ctx->sync();
+          ACU_HIP_CHECK(hipMemcpy(img_host, img.p, (size_t)s.N * s.N * 4, hipMemcpyDeviceToHost)));
+}
+
+namespace {
+int check_grad_args(acu_batch* b, const float* const* x, float eps, double* loss, const char* who) {
+  if (!b || !x || !loss) return fail(ACU_ERR_ARG, std::string(who) + ": null argument");
+  if (!(eps > 0.0f)) return fail(ACU_ERR_ARG, std::string(who) + ": eps must be > 0");
+  for (size_t i = 0; i < b->s.size(); ++i)
+    if (!x[i]) return fail(ACU_ERR_ARG, std::string(who) + ": null image pointer");
+  return ACU_OK;
+}
+}  // namespace
+
+int acu_batch_loss_grad_dev(acu_batch* b, const float* const* x_dev, const float* const* shift_dev,
+                            float lambda, float eps, float* const* grad_x_dev, float* const* grad_s_dev,
+                            float* const* curv_s_dev, double* loss) {
+  if (int rc = check_grad_args(b, x_dev, eps, loss, "acu_batch_loss_grad_dev")) return rc;
+  const size_t n = b->s.size();
+  if (grad_x_dev)
+    for (size_t i = 0; i < n; ++i)
+      if (!grad_x_dev[i]) return fail(ACU_ERR_ARG, "acu_batch_loss_grad_dev: null image-gradient pointer");
+  ACU_TRY(hipSetDevice(b->ctx->device);
+          std::vector<const float*> xs(x_dev, x_dev + n), sh(n, nullptr);
+          std::vector<float*> gx, gs(n, nullptr), hs(n, nullptr);
+          for (size_t i = 0; i < n; ++i) {
+            if (shift_dev) sh[i] = shift_dev[i];
+            if (grad_s_dev) gs[i] = grad_s_dev[i];
+            if (curv_s_dev) hs[i] = curv_s_dev[i];
+          }
+          if (grad_x_dev) gx.assign(grad_x_dev, grad_x_dev + n);
+          acu::loss_grad_run(b, xs, sh, lambda, eps, gx, gs, hs, loss));
+}
+
+int acu_batch_loss_grad(acu_batch* b, const float* const* x_host, const float* const* shift_host,
+                        float lambda, float eps, float* const* grad_x_host, float* const* grad_s_host,
+                        float* const* curv_s_host, double* loss) {
+  if (int rc = check_grad_args(b, x_host, eps, loss, "acu_batch_loss_grad")) return rc;
+  const size_t n = b->s.size();
+  if (grad_x_host)
+    for (size_t i = 0; i < n; ++i)
+      if (!grad_x_host[i]) return fail(ACU_ERR_ARG, "acu_batch_loss_grad: null image-gradient pointer");
+  ACU_TRY(hipSetDevice(b->ctx->device);
+          acu::ensure_grad(b);
+          acu_ctx* c = b->ctx;
+          std::vector<const float*> xs(n), sh(n, nullptr);
+          std::vector<float*> gx, gs(n, nullptr), hs(n, nullptr);
+          for (size_t i = 0; i < n; ++i) {
+            acu::Slice& s = b->s[i];
+            ACU_HIP_CHECK(hipMemcpyAsync(s.gx_x, x_host[i], (size_t)s.N * s.N * 4, hipMemcpyHostToDevice,
+                                         c->stream));
+            xs[i] = s.gx_x;
+            if (shift_host && shift_host[i]) {
+              ACU_HIP_CHECK(hipMemcpyAsync(s.gx_sh, shift_host[i], (size_t)s.T * 4, hipMemcpyHostToDevice,
+                                           c->stream));
+              sh[i] = s.gx_sh;
+            }
+            if (grad_s_host && grad_s_host[i]) gs[i] = s.gx_gs;
+            if (curv_s_host && curv_s_host[i]) hs[i] = s.gx_hs;
+            if (grad_x_host) gx.push_back(s.gx_g);
+          }
+          c->sync();  // host buffers are not needed past this point
+          acu::loss_grad_run(b, xs, sh, lambda, eps, gx, gs, hs, loss);
+          for (size_t i = 0; i < n; ++i) {
+            acu::Slice& s = b->s[i];
+            if (grad_x_host)
+              ACU_HIP_CHECK(hipMemcpyAsync(grad_x_host[i], s.gx_g, (size_t)s.N * s.N * 4,
+                                           hipMemcpyDeviceToHost, c->stream));
+            if (grad_s_host && grad_s_host[i])
+              ACU_HIP_CHECK(hipMemcpyAsync(grad_s_host[i], s.gx_gs, (size_t)s.T * 4, hipMemcpyDeviceToHost,
+                                           c->stream));
+            if (curv_s_host && curv_s_host[i])
+              ACU_HIP_CHECK(hipMemcpyAsync(curv_s_host[i], s.gx_hs, (size_t)s.T * 4, hipMemcpyDeviceToHost,
+                                           c->stream));
+          }
+          c->sync());
 }
 
 int acu_tvopt(acu_batch* b, const float* f_host, float mu, float lambda, int nouter, int ninner,

@@ -158,6 +158,74 @@ class Batch:
         check(self._lib.acu_iradon(self.handle, _fptr(out), _fptr(s)))
         return out
 
+    # ---- differentiation (no counterpart in the reference; built on radon) ----
+    def radon_adjoint(self, sino) -> np.ndarray:
+        """Exact transpose of :meth:`radon` (slice 0): (T,N) sinogram to (N,N) image."""
+        N, T = self.shapes[0]
+        s = _f32(sino, (T, N))
+        out = np.empty((N, N), np.float32)
+        check(self._lib.acu_radon_adjoint(self.handle, _fptr(out), _fptr(s)))
+        return out
+
+    def radon_adjoint_dev(self, i: int, img_ptr: int, sino_ptr: int) -> None:
+        check(self._lib.acu_radon_adjoint_dev(self.handle, i, img_ptr, sino_ptr))
+
+    def loss_grad(self, xs, shifts=None, lam: float = 0.1, eps: float = 1e-3,
+                  grad_x: bool = True, grad_s: bool = True, curvature: bool = False):
+        """L_i = 1/2 |shift_{s_i}(radon(x_i)) - f_i|^2 + lam TV_eps(x_i) for every
+        slice, f_i the slice's sinogram (:meth:`set_sino`).
+
+        Returns ``(loss, gx, gs)``: ``loss`` (n, 3) = (L, data term, TV_eps),
+        ``gx`` a list of (N,N) image gradients (None if not asked for), ``gs``
+        a list of (T,) shift gradients (None if not asked for); with
+        ``curvature`` a fourth item, the (T,) Gauss-Newton curvatures of the
+        data term in each shift.  ``shifts`` is None (all zero) or a list of
+        (T,) arrays / None."""
+        n = len(self.shapes)
+        if len(xs) != n:
+            raise ValueError("one image per slice")
+        keep = []
+        xp, sp, gxp, gsp, hsp = ((_F * n)() for _ in range(5))
+        gx_out, gs_out, hs_out = [], [], []
+        for i, (N, T) in enumerate(self.shapes):
+            x = _f32(xs[i], (N, N))
+            keep.append(x)
+            xp[i] = _fptr(x)
+            if shifts is not None and shifts[i] is not None:
+                sh = np.ascontiguousarray(shifts[i], np.float32)
+                if sh.shape != (T,):
+                    raise ValueError(f"shifts[{i}] must have shape ({T},)")
+                keep.append(sh)
+                sp[i] = _fptr(sh)
+            for lst, arr, shape in ((gx_out, gxp, (N, N)), (gs_out, gsp, (T,)), (hs_out, hsp, (T,))):
+                a = np.empty(shape, np.float32)
+                lst.append(a)
+                arr[i] = _fptr(a)
+        loss = np.zeros((n, 3), np.float64)
+        check(self._lib.acu_batch_loss_grad(
+            self.handle, xp, sp if shifts is not None else None, float(lam), float(eps),
+            gxp if grad_x else None, gsp if grad_s else None, hsp if curvature else None,
+            loss.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        out = (loss, gx_out if grad_x else None, gs_out if grad_s else None)
+        return out + (hs_out,) if curvature else out
+
+    def register_shifts(self, xs, shifts0=None, iters: int = 10, **kw):
+        """Per-angle centre-of-rotation shifts that align each slice's
+        sinogram with radon(x_i): Gauss-Newton steps on the data term (see
+        :func:`acumen_amd.registration.gauss_newton_shifts`).  Returns the
+        fitted (T,) float32 shift arrays and the number of evaluations (one
+        batched forward projection each)."""
+        from .registration import gauss_newton_shifts
+        if shifts0 is None:
+            shifts0 = [np.zeros(T, np.float32) for _, T in self.shapes]
+
+        def evaluate(sh):
+            _, _, gs, hs = self.loss_grad(xs, sh, lam=0.0, eps=1.0, grad_x=False, grad_s=True,
+                                          curvature=True)
+            return gs, hs
+
+        return gauss_newton_shifts(evaluate, shifts0, iters=iters, **kw)
+
     def radon_dev(self, i: int, sino_ptr: int, img_ptr: int) -> None:
         check(self._lib.acu_radon_dev(self.handle, i, sino_ptr, img_ptr))
 

@@ -136,11 +136,47 @@ int acu_get_slice(acu_batch *b, const float *f_host, float *img_fbp_host);
 int acu_tvopt(acu_batch *b, const float *f_host, float mu, float lambda, int nouter, int ninner,
               int ncg, float *img_tv_host, float *img_fbp_host, float *delta_host);
 
+/* ---- Differentiation.  No counterpart in the reference: these build on
+ * its projector (tv_hpc.F90:129-159) and the periodic differences of its
+ * tvopt (tv_hpc.F90:219-221) to make reconstruction and centre-of-rotation
+ * registration gradient-based. ---- */
+
+/* Exact transpose of radon (the same FP32 sample positions, bounds test and
+ * bilinear weights), so <radon(x), y> = <x, radon_adjoint(y)> up to FP32
+ * rounding: sino N*T in, img N*N out.  The _dev form is asynchronous on
+ * device memory of slice `slice`; the host form works on slice 0. */
+int acu_radon_adjoint_dev(acu_batch *b, int slice, float *img_dev, const float *sino_dev);
+int acu_radon_adjoint(acu_batch *b, float *img_host, const float *sino_host);
+
+/* For every slice i, with b_i its sinogram (acu_batch_set_sino) and s_i its
+ * per-angle detector shifts (T values in bins; the projection row is read
+ * at detector position d - s by linear interpolation, zero off the
+ * detector):
+ *   L_i = 1/2 |shift_{s_i}(radon(x_i)) - b_i|^2 + lambda TV_eps(x_i),
+ *   TV_eps(x) = sum_p sqrt((Dx x)_p^2 + (Dy x)_p^2 + eps^2),
+ *   (Dx u)(x,y) = u(x-1,y) - u(x,y), periodic.
+ * loss receives 3 doubles per slice: L_i, the data term and TV_eps(x_i);
+ * grad_x[i] (N*N) = dL_i/dx_i, grad_s[i] (T) = dL_i/ds_i, curv_s[i] (T) =
+ * sum_d (y(d+o) - y(d+o+1))^2 with y = radon(x_i) and o = floor(-s): the
+ * Gauss-Newton curvature of the data term in each shift (its exact second
+ * derivative between the kinks of the interpolation).  shift (or any
+ * entry) may be NULL for zero shifts; grad_x NULL skips the image gradient;
+ * grad_s / curv_s (or any entry) NULL skips that output.  eps > 0.
+ * Data terms and sums are accumulated in FP64.  Synchronous.  The _dev form
+ * takes device pointers. */
+int acu_batch_loss_grad(acu_batch *b, const float *const *x_host, const float *const *shift_host,
+                        float lambda, float eps, float *const *grad_x_host,
+                        float *const *grad_s_host, float *const *curv_s_host, double *loss);
+int acu_batch_loss_grad_dev(acu_batch *b, const float *const *x_dev, const float *const *shift_dev,
+                            float lambda, float eps, float *const *grad_x_dev,
+                            float *const *grad_s_dev, float *const *curv_s_dev, double *loss);
+
 /* Kernel timing: when enabled, every ramp-filter, backprojection, cells and
  * forward-projection launch is bracketed by HIP events on the context's
  * stream (graph mode is then run as ACU_TV_BATCHED).  acu_ctx_kernel_time
  * reports the accumulated milliseconds and launch count of one kind:
- * 0 = ramp filter, 1 = backprojection, 2 = cells pass, 3 = forward projection. */
+ * 0 = ramp filter, 1 = backprojection, 2 = cells pass, 3 = forward projection,
+ * 4 = adjoint projection. */
 int acu_ctx_set_timing(acu_ctx *ctx, int enable);
 int acu_ctx_kernel_time(acu_ctx *ctx, int kind, double *total_ms, long long *launches);
 

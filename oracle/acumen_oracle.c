@@ -15,6 +15,13 @@
  *              -ffp-contract=off), sequential FP32 reductions;
  *   orc_*_f64  the same algorithm in FP64: the accuracy yardstick.
  *
+ * Differentiation (no counterpart in the reference, built on its projector):
+ * orc_radon_adjoint_* is the exact transpose of orc_radon_*; orc_shift_* and
+ * orc_shift_adjoint_* apply a per-angle detector shift by linear
+ * interpolation and its transpose; orc_loss_grad_* evaluates
+ * 1/2 |shift(radon(x)) - b|^2 + lambda TV_eps(x) and its gradients with
+ * respect to the image and to the per-angle shifts.
+ *
  * Threading: orc_set_threads(n) parallelises radon / ramp / backprojection
  * over output elements.  Every output element is still one sequential sum,
  * so results do not depend on the thread count.

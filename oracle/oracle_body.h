@@ -199,12 +199,172 @@ void FN(radon)(int N, int T, const float *theta, const REAL *img, REAL *sino)
     free(cs);
 }
 
+/* ---- Differentiation of the projector (no counterpart in the reference;
+ * built on its radon, tv_hpc.F90:129-159) ---- */
+
+/* Exact transpose of FN(radon): the same sample positions, bounds test and
+ * bilinear weights (tv_hpc.F90:143-153), scattered into the image instead of
+ * gathered, so that <radon(x), y> = <x, radon_adjoint(y)>.  Angles are split
+ * over threads, each with a private image, summed at the end. */
+void FN(radon_adjoint)(int N, int T, const float *theta, const REAL *sino, REAL *img)
+{
+    REAL *sn = (REAL *)malloc(sizeof(REAL) * (size_t)T);
+    REAL *cs = (REAL *)malloc(sizeof(REAL) * (size_t)T);
+    FN(angles)(T, theta, 0, sn, cs);
+    const REAL m = FN(midpoint)(N);
+    const size_t NN = (size_t)N * N;
+    const int nt = g_threads > T ? (T > 0 ? T : 1) : g_threads;
+    REAL *part = (REAL *)calloc((size_t)nt * NN, sizeof(REAL));
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
+    for (int k = 0; k < nt; ++k) {
+        REAL *acc = part + (size_t)k * NN;
+        for (int jt = 1 + k; jt <= T; jt += nt) {
+            const REAL s = sn[jt - 1], c = cs[jt - 1];
+            for (int jy = 1; jy <= N; ++jy) {
+                const REAL sv = sino[(jy - 1) + (size_t)(jt - 1) * N] / (REAL)N;
+                for (int jx = 1; jx <= N; ++jx) {
+                    const REAL xr = (c * ((REAL)jx - m) - s * ((REAL)jy - m)) + m;
+                    const REAL yr = (s * ((REAL)jx - m) + c * ((REAL)jy - m)) + m;
+                    const int y1 = (int)FLOOR(yr), y2 = y1 + 1;
+                    const int x1 = (int)FLOOR(xr), x2 = x1 + 1;
+                    const int mn = x1 < y1 ? x1 : y1;
+                    const int mx = x2 > y2 ? x2 : y2;
+                    if (mn > 1 && mx < N + 1) {
+                        const REAL ax = (REAL)x2 - xr, bx = xr - (REAL)x1;
+                        const REAL ay = (REAL)y2 - yr, by = yr - (REAL)y1;
+                        acc[(x1 - 1) + (size_t)(y1 - 1) * N] += (sv * ax) * ay;
+                        acc[(x2 - 1) + (size_t)(y1 - 1) * N] += (sv * bx) * ay;
+                        acc[(x1 - 1) + (size_t)(y2 - 1) * N] += (sv * ax) * by;
+                        acc[(x2 - 1) + (size_t)(y2 - 1) * N] += (sv * bx) * by;
+                    }
+                }
+            }
+        }
+    }
+    for (size_t i = 0; i < NN; ++i) {
+        REAL v = RC(0.0);
+        for (int k = 0; k < nt; ++k) v += part[(size_t)k * NN + i];
+        img[i] = v;
+    }
+    free(part);
+    free(sn);
+    free(cs);
+}
+
+/* Per-angle centre-of-rotation shift of one projection row: out(d) is the
+ * row sampled at detector position d - s by linear interpolation, zero
+ * outside 1..N.  With o = floor(-s) and w = -s - o (exact in FP32):
+ *   out(d) = (1 - w) y(d + o) + w y(d + o + 1). */
+static void FN(shift_weights)(float s, int *o, REAL *w)
+{
+    const REAL sg = -(REAL)s;
+    const REAL fl = FLOOR(sg);
+    *o = (int)fl;
+    *w = sg - fl;
+}
+
+static inline REAL FN(at)(const REAL *row, int N, int i) { return (i >= 1 && i <= N) ? row[i - 1] : RC(0.0); }
+
+void FN(shift)(int N, int T, const float *shift, const REAL *sino, REAL *out)
+{
+    for (int jt = 0; jt < T; ++jt) {
+        int o; REAL w;
+        FN(shift_weights)(shift[jt], &o, &w);
+        const REAL *row = sino + (size_t)jt * N;
+        for (int d = 1; d <= N; ++d)
+            out[(size_t)jt * N + d - 1] = (RC(1.0) - w) * FN(at)(row, N, d + o) + w * FN(at)(row, N, d + o + 1);
+    }
+}
+
+/* Transpose of FN(shift): out(k) = (1 - w) r(k - o) + w r(k - o - 1). */
+void FN(shift_adjoint)(int N, int T, const float *shift, const REAL *sino, REAL *out)
+{
+    for (int jt = 0; jt < T; ++jt) {
+        int o; REAL w;
+        FN(shift_weights)(shift[jt], &o, &w);
+        const REAL *row = sino + (size_t)jt * N;
+        for (int k = 1; k <= N; ++k)
+            out[(size_t)jt * N + k - 1] = (RC(1.0) - w) * FN(at)(row, N, k - o) + w * FN(at)(row, N, k - o - 1);
+    }
+}
+
 #endif /* ORC_TV_ONLY */
 
 /* Periodic neighbour indices of tvopt (tv_hpc.F90:220,234-237), derived from
  * FP32 floor formulas that reduce exactly to wrap-around for N < 2^24. */
 static inline int TVFN(nxt)(int j, int N) { return j == N ? 1 : j + 1; }
 static inline int TVFN(prv)(int j, int N) { return j == 1 ? N : j - 1; }
+
+#ifndef ORC_TV_ONLY
+/* Objective of gradient-based reconstruction and registration:
+ *   L(x, s) = 1/2 |shift_s(radon(x)) - b|^2 + lambda TV_eps(x),
+ *   TV_eps(x) = sum_p sqrt((Dx x)_p^2 + (Dy x)_p^2 + eps^2),
+ * with the module's periodic differences (Dx u)(x,y) = u(x-1,y) - u(x,y)
+ * (tv_hpc.F90:219-221).  Gradients:
+ *   dL/dx   = radon_adjoint(shift_adjoint(r)) + lambda (Dx^T qx + Dy^T qy),
+ *             r = shift_s(radon(x)) - b, q = Dx / sqrt(.), Dx^T v = v(x+1) - v(x);
+ *   dL/ds_t = sum_d r(d,t) (y(d+o,t) - y(d+o+1,t)),  y = radon(x),
+ *   hs_t    = sum_d (y(d+o,t) - y(d+o+1,t))^2, the Gauss-Newton curvature
+ *             of the data term in s_t (its exact second derivative between
+ *             the kinks of the interpolation, where o is constant).
+ * loss[0..2] = total, data term, TV_eps; shift, gx, gs, hs may be NULL. */
+void FN(loss_grad)(int N, int T, const float *theta, const REAL *x, const float *shift,
+                   const REAL *b, double lambda, double eps, double *loss, REAL *gx, REAL *gs,
+                   REAL *hs)
+{
+    const size_t NN = (size_t)N * N, NT = (size_t)N * T;
+    REAL *y = (REAL *)malloc(sizeof(REAL) * NT);
+    REAL *r = (REAL *)malloc(sizeof(REAL) * NT);
+    FN(radon)(N, T, theta, x, y);
+    double data = 0.0;
+    for (int jt = 0; jt < T; ++jt) {
+        int o = 0; REAL w = RC(0.0);
+        if (shift) FN(shift_weights)(shift[jt], &o, &w);
+        const REAL *row = y + (size_t)jt * N;
+        double g = 0.0, h = 0.0;
+        for (int d = 1; d <= N; ++d) {
+            const REAL ya = FN(at)(row, N, d + o), yb = FN(at)(row, N, d + o + 1);
+            const REAL rv = ((RC(1.0) - w) * ya + w * yb) - b[(size_t)jt * N + d - 1];
+            r[(size_t)jt * N + d - 1] = rv;
+            data += 0.5 * (double)rv * (double)rv;
+            g += (double)rv * ((double)ya - (double)yb);
+            h += ((double)ya - (double)yb) * ((double)ya - (double)yb);
+        }
+        if (gs) gs[jt] = (REAL)g;
+        if (hs) hs[jt] = (REAL)h;
+    }
+    double tv = 0.0;
+    REAL *qx = (REAL *)malloc(sizeof(REAL) * NN);
+    REAL *qy = (REAL *)malloc(sizeof(REAL) * NN);
+    for (int jy = 1; jy <= N; ++jy)
+        for (int jx = 1; jx <= N; ++jx) {
+            const size_t i = (size_t)(jx - 1) + (size_t)(jy - 1) * N;
+            const REAL dx = x[(size_t)(TVFN(prv)(jx, N) - 1) + (size_t)(jy - 1) * N] - x[i];
+            const REAL dy = x[(size_t)(jx - 1) + (size_t)(TVFN(prv)(jy, N) - 1) * N] - x[i];
+            const REAL n = SQRT(dx * dx + dy * dy + (REAL)(eps * eps));
+            tv += (double)n;
+            qx[i] = dx / n;
+            qy[i] = dy / n;
+        }
+    if (gx) {
+        REAL *rs = y;  /* y is no longer needed */
+        if (shift) FN(shift_adjoint)(N, T, shift, r, rs);
+        else memcpy(rs, r, sizeof(REAL) * NT);
+        FN(radon_adjoint)(N, T, theta, rs, gx);
+        for (int jy = 1; jy <= N; ++jy)
+            for (int jx = 1; jx <= N; ++jx) {
+                const size_t i = (size_t)(jx - 1) + (size_t)(jy - 1) * N;
+                const REAL t = (qx[(size_t)(TVFN(nxt)(jx, N) - 1) + (size_t)(jy - 1) * N] - qx[i])
+                             + (qy[(size_t)(jx - 1) + (size_t)(TVFN(nxt)(jy, N) - 1) * N] - qy[i]);
+                gx[i] = gx[i] + (REAL)lambda * t;
+            }
+    }
+    loss[0] = data + lambda * tv;
+    loss[1] = data;
+    loss[2] = tv;
+    free(qx); free(qy); free(y); free(r);
+}
+#endif /* ORC_TV_ONLY */
 
 /* Split-Bregman TV with inner CG, tv_hpc.F90:162-363.
  *   f         in  (N,T) windowed sinogram (module array f)

@@ -53,6 +53,12 @@ def lib():
             getattr(_lib, f"orc_tvopt_{p}").argtypes = [
                 ctypes.c_int, ctypes.c_int, _F, ctypes.c_int, _F, real, real,
                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ptr, ptr, ptr]
+            getattr(_lib, f"orc_radon_adjoint_{p}").argtypes = [ctypes.c_int, ctypes.c_int, _F, ptr, ptr]
+            getattr(_lib, f"orc_shift_{p}").argtypes = [ctypes.c_int, ctypes.c_int, _F, ptr, ptr]
+            getattr(_lib, f"orc_shift_adjoint_{p}").argtypes = [ctypes.c_int, ctypes.c_int, _F, ptr, ptr]
+            getattr(_lib, f"orc_loss_grad_{p}").argtypes = [
+                ctypes.c_int, ctypes.c_int, _F, ptr, _F, ptr, ctypes.c_double, ctypes.c_double,
+                _D, ptr, ptr, ptr]
         _lib.orc_tvopt_f32d.argtypes = [
             ctypes.c_int, ctypes.c_int, _F, ctypes.c_int, _F, ctypes.c_float, ctypes.c_float,
             ctypes.c_int, ctypes.c_int, ctypes.c_int, _F, _F, _F]
@@ -198,3 +204,73 @@ def pass_batch_f32(imgs: np.ndarray, theta, wpu: int, threads: int):
     finally:
         set_threads(1)
     return sinos, out
+
+
+# ---- differentiation (no counterpart in the reference; built on its radon) ----
+
+def radon_adjoint(sino: np.ndarray, theta, prec="f32") -> np.ndarray:
+    """Exact transpose of :func:`radon`: (T,N) sinogram to (N,N) image."""
+    T, N = sino.shape
+    th = _theta(theta)
+    assert th.size == T
+    s = np.ascontiguousarray(sino, _dt(prec))
+    img = np.empty((N, N), _dt(prec))
+    getattr(lib(), f"orc_radon_adjoint_{prec}")(N, T, th.ctypes.data_as(_F), _p(s), _p(img))
+    return img
+
+
+def _shifts(shift, T):
+    sh = np.ascontiguousarray(shift, np.float32)
+    assert sh.shape == (T,)
+    return sh
+
+
+def shift(sino: np.ndarray, shifts, prec="f32") -> np.ndarray:
+    """Each projection row sampled at detector position d - s_t (linear
+    interpolation, zero outside the detector)."""
+    T, N = sino.shape
+    sh = _shifts(shifts, T)
+    s = np.ascontiguousarray(sino, _dt(prec))
+    out = np.empty_like(s)
+    getattr(lib(), f"orc_shift_{prec}")(N, T, sh.ctypes.data_as(_F), _p(s), _p(out))
+    return out
+
+
+def shift_adjoint(sino: np.ndarray, shifts, prec="f32") -> np.ndarray:
+    """Transpose of :func:`shift`."""
+    T, N = sino.shape
+    sh = _shifts(shifts, T)
+    s = np.ascontiguousarray(sino, _dt(prec))
+    out = np.empty_like(s)
+    getattr(lib(), f"orc_shift_adjoint_{prec}")(N, T, sh.ctypes.data_as(_F), _p(s), _p(out))
+    return out
+
+
+def loss_grad(x: np.ndarray, theta, shifts, b: np.ndarray, lam: float, eps: float,
+              prec="f64", grad=True, curvature=False):
+    """L = 1/2 |shift_s(radon(x)) - b|^2 + lam * TV_eps(x) and its gradients.
+
+    Returns (loss[3] = total, data, TV_eps; dL/dx (N,N); dL/ds (T,)), the
+    gradients None when ``grad`` is False, plus the Gauss-Newton curvature
+    of the data term in each shift (T,) as a fourth item when
+    ``curvature`` is True.  ``shifts`` may be None (zero)."""
+    N = x.shape[0]
+    th = _theta(theta)
+    T = th.size
+    dt = _dt(prec)
+    xi = np.ascontiguousarray(x, dt)
+    bi = np.ascontiguousarray(b, dt)
+    assert bi.shape == (T, N)
+    loss = np.zeros(3, np.float64)
+    gx = np.empty((N, N), dt) if grad else None
+    gs = np.empty(T, dt) if grad else None
+    hs = np.empty(T, dt) if curvature else None
+    sh = None if shifts is None else _shifts(shifts, T)
+    getattr(lib(), f"orc_loss_grad_{prec}")(
+        N, T, th.ctypes.data_as(_F), _p(xi), None if sh is None else sh.ctypes.data_as(_F),
+        _p(bi), float(lam), float(eps), loss.ctypes.data_as(_D),
+        None if gx is None else _p(gx), None if gs is None else _p(gs),
+        None if hs is None else _p(hs))
+    if curvature:
+        return loss, gx, gs, hs
+    return loss, gx, gs
