@@ -170,6 +170,9 @@ int fp_chunks(int N, int clen);
 int fp_sum_blocks(int N, int T);
 int fpl_angles_per_lane(int N, int T);
 int fpl_blocks(int N, int T, int al);
+// true when some chunk of the LDS projector can outgrow its box and read
+// the global bilinear cells (which then have to be built first)
+bool fpl_may_overflow(int N, int T, const float* theta_deg, int al);
 int cell_blocks(int N);
 
 }  // namespace acu

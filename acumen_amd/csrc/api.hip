@@ -150,6 +150,7 @@ struct Slice {
   float* fpart = nullptr;  // per-chunk partial ray sums of the forward projector
   int fp_clen = FP_CHUNK_DEFAULT;
   int fpl_al = 1;
+  bool fpl_cells = true;  // the LDS projector may need the global cells of this slice
   // TV arena
   float *uk = nullptr, *ukp = nullptr, *dkx = nullptr, *dky = nullptr, *bkx = nullptr,
         *bky = nullptr, *ifkt = nullptr, *iukt = nullptr, *r = nullptr, *p = nullptr,
@@ -292,6 +293,7 @@ void upload_theta(acu_batch* b, int i) {
   make_angles(s.T, s.theta.data(), false, fp.data());
   ACU_HIP_CHECK(hipMemcpy(s.scbp, bp.data(), bp.size() * sizeof(float2), hipMemcpyHostToDevice));
   ACU_HIP_CHECK(hipMemcpy(s.scfp, fp.data(), fp.size() * sizeof(float2), hipMemcpyHostToDevice));
+  s.fpl_cells = fpl_may_overflow(s.N, s.T, s.theta.data(), s.fpl_al);
   s.theta_set = true;
 }
 
@@ -394,11 +396,21 @@ void iradon_seq(acu_batch* b, const DevArray<RFTask>& rf, const DevArray<BPTask>
   check_launch();
 }
 
+// The gather projector always reads the cells; the LDS projector only where
+// a box can overflow (fpl_may_overflow).
+bool needs_cells(acu_batch* b, int sl) {
+  if (!b->fp_lds) return true;
+  if (sl >= 0) return b->s[sl].fpl_cells;
+  for (auto& s : b->s)
+    if (s.fpl_cells) return true;
+  return false;
+}
+
 void radon_seq(acu_batch* b, const DevArray<CellTask>& cl, const DevArray<FPTask>& fp,
                const DevArray<FPLTask>& fl, int sl) {
   acu_ctx* c = b->ctx;
   const Range r1 = cl_range(b, sl), r2 = fp_range(b, sl), r3 = fs_range(b, sl), r4 = fl_range(b, sl);
-  timed(c, K_CELLS, [&] { launch_cells(cl.d, cl.n, r1.base, r1.n, c->stream); });
+  if (needs_cells(b, sl)) timed(c, K_CELLS, [&] { launch_cells(cl.d, cl.n, r1.base, r1.n, c->stream); });
   if (b->fp_lds)
     timed(c, K_FP, [&] { launch_forward_lds(fl.d, fl.n, r4.base, r4.n, b->fpl_al, c->stream); });
   else
@@ -535,7 +547,7 @@ Slice& slice_of(acu_batch* b, int sl) {
 void radon_dev(acu_batch* b, int sl, float* sino, const float* img) {
   Slice& s = slice_of(b, sl);
   acu_ctx* c = b->ctx;
-  {
+  if (needs_cells(b, sl)) {
     CellTask t = cell_task(s, img);
     t.block_begin = 0;
     CellTask* d = put_adhoc(b, t);
@@ -620,10 +632,12 @@ void ensure_grad(acu_batch* b) {
 void radon_many(acu_batch* b, const std::vector<const float*>& imgs, const std::vector<float*>& sinos) {
   acu_ctx* c = b->ctx;
   const int n = (int)b->s.size();
-  std::vector<CellTask> cl(n);
-  for (int i = 0; i < n; ++i) cl[i] = cell_task(b->s[i], imgs[i]);
-  b->cl_g.upload(cl);
-  timed(c, K_CELLS, [&] { launch_cells(b->cl_g.d, n, 0, b->cl_total, c->stream); });
+  if (needs_cells(b, -1)) {
+    std::vector<CellTask> cl(n);
+    for (int i = 0; i < n; ++i) cl[i] = cell_task(b->s[i], imgs[i]);
+    b->cl_g.upload(cl);
+    timed(c, K_CELLS, [&] { launch_cells(b->cl_g.d, n, 0, b->cl_total, c->stream); });
+  }
   if (b->fp_lds) {
     std::vector<FPLTask> fl(n);
     for (int i = 0; i < n; ++i) fl[i] = fpl_task(b->s[i], imgs[i], sinos[i]);
@@ -841,7 +855,10 @@ int acu_batch_set_theta(acu_batch* b, int slice, const float* theta_deg) {
     return fail(ACU_ERR_ARG, "acu_batch_set_theta: bad argument");
   ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync();
           std::memcpy(b->s[slice].theta.data(), theta_deg, sizeof(float) * b->s[slice].T);
-          acu::upload_theta(b, slice));
+          acu::upload_theta(b, slice);
+          // whether the cells pass is launched depends on the angles: a
+          // captured outer iteration is rebuilt on the next tvopt
+          if (b->graph) { ACU_HIP_CHECK(hipGraphExecDestroy(b->graph)); b->graph = nullptr; });
 }
 
 int acu_batch_set_sino(acu_batch* b, int slice, const float* f_host) {

@@ -28,6 +28,9 @@
 //    the bilinear cells of fp.hip from global memory (exact positions).
 #include "acu_internal.hpp"
 
+#include <algorithm>
+#include <cmath>
+
 namespace acu {
 namespace {
 
@@ -279,6 +282,31 @@ int fpl_angles_per_lane(int N, int T) {
   while (al < AMAX && (4.0 * (2 * al) - 1.0) * dth * r <= room) al *= 2;
   return al;
 }
+bool fpl_may_overflow(int N, int T, const float* theta_deg, int al) {
+  // The box of a chunk spans (per axis) at most the 64-ray x FPL_U-step
+  // footprint of one angle, hypot(63, FPL_U - 1), plus the drift of a
+  // sample between the first and last angle of its block, span * r with r
+  // the largest distance of an in-image sample from the centre, plus the
+  // kernel's margins (2 + 4 pixels and the floors).  When that fits the box
+  // for every angle block, no chunk takes the global-cells path.
+  const double pi = 3.141592653589793;
+  const double r = 0.70711 * ((double)N + 2.0);
+  const double foot = std::hypot(63.0, (double)(FPL_U - 1));
+  const int ab = 4 * al;
+  double span = 0.0;
+  for (int a0 = 0; a0 < T; a0 += ab) {
+    double lo = 1e300, hi = -1e300;
+    for (int a = a0; a < std::min(T, a0 + ab); ++a) {
+      const double ct = pi * ((double)theta_deg[a] - 90.0) / 180.0;
+      lo = std::min(lo, ct);
+      hi = std::max(hi, ct);
+    }
+    span = std::max(span, hi - lo);
+  }
+  const double extent = foot + (span + 1e-6) * r + 8.0;
+  return extent > (double)std::min(FPL_PITCH, FPL_CAP / FPL_PITCH);
+}
+
 int fpl_blocks(int N, int T, int al) {
   const int nrb = (N + 63) / 64;
   const int nab = (T + 4 * al - 1) / (4 * al);

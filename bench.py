@@ -62,10 +62,15 @@ def make_inputs(ctx, batch, N, T, seed0):
     return th, wpu, sinos
 
 
+def fp_kernel_name():
+    """The forward-projection kernel the library launches (ACUMEN_FP_PATH)."""
+    return "fp_kernel" if os.environ.get("ACUMEN_FP_PATH") == "gather" else "fpl_kernel"
+
+
 def traffic_from_profile():
     """HBM bytes per forward-projection launch from the committed rocprofv3
     counter summary, if one is present."""
-    path = os.path.join(ROOT, "profiles", "fp_kernel_traffic_2049x1500.txt")
+    path = os.path.join(ROOT, "profiles", f"{fp_kernel_name()}_traffic_2049x1500.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -105,6 +110,8 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-slice leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--nouter", type=int, default=NOUTER,
+                    help="outer iterations (default: the workload's 50; fewer only for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -132,14 +139,14 @@ def main() -> int:
 
     for w in range(args.warmup):
         log(f"warmup step {w + 1}/{args.warmup}")
-        batch.tvopt(MU, LAM, NOUTER, NINNER, NCG, ACU_TV_BATCHED)
+        batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_BATCHED)
         ctx.synchronize()
 
     ctx.set_timing(True)
     ctx.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        batch.tvopt(MU, LAM, NOUTER, NINNER, NCG, ACU_TV_BATCHED)
+        batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_BATCHED)
         ctx.synchronize()
         log(f"step {k + 1}/{args.steps} done at {time.perf_counter() - t0:.2f} s")
     elapsed = time.perf_counter() - t0
@@ -171,7 +178,7 @@ def main() -> int:
             one.set_sino(0, sinos[0])
             ctx.synchronize()
             t2 = time.perf_counter()
-            one.tvopt(MU, LAM, NOUTER, NINNER, NCG, ACU_TV_PLAIN)
+            one.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_PLAIN)
             ctx.synchronize()
             plain_rate = 1.0 / (time.perf_counter() - t2)
     batch.close()
@@ -184,7 +191,7 @@ def main() -> int:
     traffic = traffic_from_profile()
     roofline = {
         "bound": "mfma",
-        "kernel": "fp_kernel (ray-driven bilinear projection on the FP32 VALU; compute roofline)",
+        "kernel": f"{fp_kernel_name()} (ray-driven bilinear projection on the FP32 VALU; compute roofline)",
         "achieved": round(achieved, 3),
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
@@ -216,9 +223,9 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"split-Bregman TV, {B} slices of {N_DET}^2 (N={N}) x {T} angles, "
-                               f"I0=5e3, mu={MU}, lambda={LAM}, {NOUTER} outer x {NCG} CG, "
+                               f"I0=5e3, mu={MU}, lambda={LAM}, {args.nouter} outer x {NCG} CG, "
                                "FBP warm start, batched path",
-                   "slices_per_step": B, "nimage": N, "ntheta": T, "nouter": NOUTER,
+                   "slices_per_step": B, "nimage": N, "ntheta": T, "nouter": args.nouter,
                    "ncg": NCG, "wpu": wpu},
         "plain_slices_per_s": None if plain_rate is None else round(plain_rate, 5),
         "batched_vs_plain": None if plain_rate is None else round(value / plain_rate, 3),

@@ -46,6 +46,20 @@ def test_radon_matches_oracle(ctx, N, T):
     assert rel(s, ref.radon(x, th, "f32")) < FBP_TOL
 
 
+@pytest.mark.parametrize("N,T", [(1025, 60), (2049, 150)])
+def test_radon_boxes_too_large_for_lds(ctx, N, T):
+    # few angles: an angle block drifts farther than the LDS box allows, so
+    # chunks away from the centre read the global bilinear cells
+    say("radon global-cells path", N, T)
+    th = synth.angles(T)
+    x = disk_phantom(N, 12, N + T)
+    with plan(ctx, N, T, th) as b:
+        s = b.radon(x)
+    assert rel(s, ref.radon(x, th, "f32")) < FBP_TOL
+    with ref.fp32_angles():
+        assert rel(s, ref.radon(x, th, "f64")) < FBP_TOL
+
+
 def test_radon_random_angles_and_image(ctx):
     N, T = 129, 97
     say("radon random", N, T)
