@@ -55,6 +55,9 @@ SIGNATURES = {
     "acu_iradon": (_c_int, [_vp, _fp, _fp]),
     "acu_get_slice": (_c_int, [_vp, _fp, _fp]),
     "acu_tvopt": (_c_int, [_vp, _fp, _c_float, _c_float, _c_int, _c_int, _c_int, _fp, _fp, _fp]),
+    "acu_batch_autotune": (_c_int, [_vp, _c_int, ctypes.c_char_p, _c_int]),
+    "acu_batch_fp_variant": (_c_int, [_vp, _c_int]),
+    "acu_batch_set_fp_variant": (_c_int, [_vp, _c_int, _c_int]),
     "acu_radon_adjoint_dev": (_c_int, [_vp, _c_int, _vp, _vp]),
     "acu_radon_adjoint": (_c_int, [_vp, _fp, _fp]),
     "acu_batch_loss_grad": (_c_int, [_vp, ctypes.POINTER(_fp), ctypes.POINTER(_fp), _c_float, _c_float,

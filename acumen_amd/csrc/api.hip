@@ -5,6 +5,10 @@
 #include "acu_grad.hpp"
 #include "acumen.h"
 
+#include <map>
+#include <sstream>
+#include <tuple>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -149,7 +153,9 @@ struct Slice {
   float4 *cells = nullptr, *cellsT = nullptr;
   float* fpart = nullptr;  // per-chunk partial ray sums of the forward projector
   int fp_clen = FP_CHUNK_DEFAULT;
-  int fpl_al = 1;
+  // forward-projection variant: 0 = global gathers of bilinear cells
+  // (fp.hip), 1 / 2 / 4 = LDS boxes with that many angles per lane (fpl.hip)
+  int fpv = 2;
   bool fpl_cells = true;  // the LDS projector may need the global cells of this slice
   // TV arena
   float *uk = nullptr, *ukp = nullptr, *dkx = nullptr, *dky = nullptr, *bkx = nullptr,
@@ -170,6 +176,21 @@ struct Slice {
       fp_begin = 0, fp_n = 0, fs_begin = 0, fs_n = 0, fl_begin = 0, fl_n = 0, ew_begin = 0, ew_n = 0, sw_begin = 0, sw_n = 0;
 };
 
+// Task arrays of one radon operand pair, one array per variant group; a
+// slice appears in the cells array only when its variant reads the cells.
+struct FPSet {
+  DevArray<CellTask> cl;
+  DevArray<FPLTask> fl[3];
+  DevArray<FPTask> fp;
+  void release() {
+    cl.release();
+    for (auto& a : fl) a.release();
+    fp.release();
+  }
+};
+
+inline int fl_group(int al) { return al >= 4 ? 2 : al == 2 ? 1 : 0; }
+
 }  // namespace
 }  // namespace acu
 
@@ -179,25 +200,20 @@ struct acu_batch {
   void* ops_mem = nullptr;
   void* tv_mem = nullptr;
   int tv_delta_cap = 0;
-  int rf_total = 0, bp_total = 0, cl_total = 0, fp_total = 0, fs_total = 0, fl_total = 0,
-      ew_total = 0, sw_total = 0;
-  bool fp_lds = true;  // forward projection path: LDS boxes (fpl.hip) or global gathers (fp.hip)
-  int fpl_al = 4;      // angles per lane of the LDS forward projector, batch-wide
+  int rf_total = 0, bp_total = 0, cl_total = 0, fp_total = 0, fs_total = 0, ew_total = 0,
+      sw_total = 0;
+  int fl_total[3] = {0, 0, 0};  // LDS projector blocks per angles-per-lane group (1, 2, 4)
   // task sets for the fixed operand combinations of FBP / TV
   acu::DevArray<acu::RFTask> rf_f, rf_fk, rf_ruk, rf_rp;
   acu::DevArray<acu::BPTask> bp_fbp, bp_ifkt, bp_iukt, bp_irpt;
-  acu::DevArray<acu::CellTask> cl_uk, cl_p;
-  acu::DevArray<acu::FPTask> fp_ruk, fp_rp;
-  acu::DevArray<acu::FPLTask> fl_ruk, fl_rp;
+  acu::FPSet fp_uk, fp_p;  // radon(uk) -> ruk, radon(p) -> rp
   acu::DevArray<acu::TVTask> tv;
   float tv_mu = 0.f, tv_lambda = 0.f;
   // single-slot task buffer for calls on caller-provided device pointers
   void* adhoc = nullptr;
   // differentiation: arena and per-call task sets
   void* grad_mem = nullptr;
-  acu::DevArray<acu::CellTask> cl_g;
-  acu::DevArray<acu::FPLTask> fl_g;
-  acu::DevArray<acu::FPTask> fp_g;
+  acu::FPSet fp_g;
   acu::DevArray<acu::ResTask> res_g;
   acu::DevArray<acu::AdjTask> adj_g;
   acu::DevArray<acu::TVGTask> tvg_g;
@@ -210,9 +226,8 @@ struct acu_batch {
     if (graph) hipGraphExecDestroy(graph);
     rf_f.release(); rf_fk.release(); rf_ruk.release(); rf_rp.release();
     bp_fbp.release(); bp_ifkt.release(); bp_iukt.release(); bp_irpt.release();
-    cl_uk.release(); cl_p.release(); fp_ruk.release(); fp_rp.release(); tv.release();
-    fl_ruk.release(); fl_rp.release();
-    cl_g.release(); fl_g.release(); fp_g.release(); res_g.release(); adj_g.release();
+    fp_uk.release(); fp_p.release(); tv.release();
+    fp_g.release(); res_g.release(); adj_g.release();
     tvg_g.release(); loss_g.release();
     if (grad_mem) hipFree(grad_mem);
     if (adhoc) hipFree(adhoc);
@@ -293,7 +308,6 @@ void upload_theta(acu_batch* b, int i) {
   make_angles(s.T, s.theta.data(), false, fp.data());
   ACU_HIP_CHECK(hipMemcpy(s.scbp, bp.data(), bp.size() * sizeof(float2), hipMemcpyHostToDevice));
   ACU_HIP_CHECK(hipMemcpy(s.scfp, fp.data(), fp.size() * sizeof(float2), hipMemcpyHostToDevice));
-  s.fpl_cells = fpl_may_overflow(s.N, s.T, s.theta.data(), s.fpl_al);
   s.theta_set = true;
 }
 
@@ -304,7 +318,7 @@ BPTask bp_task(const Slice& s, const float* in, float* out) {
   return BPTask{s.N, s.T, s.mid, s.bp_begin, s.ntx, s.nty, in, s.cmask, s.scbp, s.tact, out};
 }
 FPLTask fpl_task(const Slice& s, const float* img, float* out) {
-  return FPLTask{s.N, s.T, s.mid, s.fl_begin, s.fpl_al, (s.T + 4 * s.fpl_al - 1) / (4 * s.fpl_al),
+  return FPLTask{s.N, s.T, s.mid, s.fl_begin, s.fpv, (s.T + 4 * s.fpv - 1) / (4 * s.fpv),
                  img, s.cells, s.scfp, out};
 }
 CellTask cell_task(const Slice& s, const float* img) {
@@ -322,6 +336,53 @@ void build(acu_batch* b, DevArray<T>& arr, F&& make) {
   arr.upload(v);
 }
 
+// Block ranges of the forward-projection groups from each slice's variant:
+// cells only for slices whose variant reads them, LDS-projector blocks
+// within the slice's angles-per-lane group, gather blocks within the
+// gather group.
+void assign_fp(acu_batch* b) {
+  int cl = 0, fp = 0, fs = 0, fl[3] = {0, 0, 0};
+  for (auto& s : b->s) {
+    s.fpl_cells = s.fpv == 0 || !s.theta_set || fpl_may_overflow(s.N, s.T, s.theta.data(), s.fpv);
+    s.cl_begin = cl; s.cl_n = s.fpl_cells ? cell_blocks(s.N) : 0; cl += s.cl_n;
+    s.fl_begin = 0; s.fl_n = 0; s.fp_begin = fp; s.fp_n = 0; s.fs_begin = fs; s.fs_n = 0;
+    if (s.fpv > 0) {
+      const int g = fl_group(s.fpv);
+      s.fl_begin = fl[g]; s.fl_n = fpl_blocks(s.N, s.T, s.fpv); fl[g] += s.fl_n;
+    } else {
+      s.fp_n = fp_ray_blocks(s.N) * s.T * fp_chunks(s.N, s.fp_clen); fp += s.fp_n;
+      s.fs_n = fp_sum_blocks(s.N, s.T); fs += s.fs_n;
+    }
+  }
+  b->cl_total = cl; b->fp_total = fp; b->fs_total = fs;
+  for (int g = 0; g < 3; ++g) b->fl_total[g] = fl[g];
+}
+
+template <class ImgF, class OutF>
+void build_fpset(acu_batch* b, FPSet& set, ImgF img, OutF out) {
+  std::vector<CellTask> cl;
+  std::vector<FPLTask> fl[3];
+  std::vector<FPTask> fp;
+  for (auto& s : b->s) {
+    if (s.cl_n > 0) cl.push_back(cell_task(s, img(s)));
+    if (s.fpv > 0) fl[fl_group(s.fpv)].push_back(fpl_task(s, img(s), out(s)));
+    else fp.push_back(fp_task(s, out(s)));
+  }
+  set.cl.upload(cl);
+  for (int g = 0; g < 3; ++g) set.fl[g].upload(fl[g]);
+  set.fp.upload(fp);
+}
+
+void build_tv_tasks(acu_batch* b);
+
+// After a change of variants or angles: new ranges, new task arrays, and a
+// captured outer iteration is rebuilt on the next tvopt.
+void refresh_fp(acu_batch* b) {
+  assign_fp(b);
+  if (b->tv_mem) build_tv_tasks(b);
+  if (b->graph) { ACU_HIP_CHECK(hipGraphExecDestroy(b->graph)); b->graph = nullptr; }
+}
+
 void build_op_tasks(acu_batch* b) {
   build(b, b->rf_f, [](Slice& s) { return rf_task(s, s.f, s.sinoCS); });
   build(b, b->bp_fbp, [](Slice& s) { return bp_task(s, s.sinoCS, s.fbp); });
@@ -334,12 +395,8 @@ void build_tv_tasks(acu_batch* b) {
   build(b, b->bp_ifkt, [](Slice& s) { return bp_task(s, s.sinoCS, s.ifkt); });
   build(b, b->bp_iukt, [](Slice& s) { return bp_task(s, s.sinoCS, s.iukt); });
   build(b, b->bp_irpt, [](Slice& s) { return bp_task(s, s.sinoCS, s.irpt); });
-  build(b, b->cl_uk, [](Slice& s) { return cell_task(s, s.uk); });
-  build(b, b->cl_p, [](Slice& s) { return cell_task(s, s.p); });
-  build(b, b->fp_ruk, [](Slice& s) { return fp_task(s, s.ruk); });
-  build(b, b->fp_rp, [](Slice& s) { return fp_task(s, s.rp); });
-  build(b, b->fl_ruk, [](Slice& s) { return fpl_task(s, s.uk, s.ruk); });
-  build(b, b->fl_rp, [](Slice& s) { return fpl_task(s, s.p, s.rp); });
+  build_fpset(b, b->fp_uk, [](Slice& s) { return (const float*)s.uk; }, [](Slice& s) { return s.ruk; });
+  build_fpset(b, b->fp_p, [](Slice& s) { return (const float*)s.p; }, [](Slice& s) { return s.rp; });
 }
 
 void build_tv_task_array(acu_batch* b) {
@@ -381,10 +438,6 @@ void require_theta(acu_batch* b) {
 struct Range { int base, n; };
 Range rf_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->rf_total} : Range{b->s[sl].rf_begin, b->s[sl].rf_n}; }
 Range bp_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->bp_total} : Range{b->s[sl].bp_begin, b->s[sl].bp_n}; }
-Range cl_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->cl_total} : Range{b->s[sl].cl_begin, b->s[sl].cl_n}; }
-Range fp_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->fp_total} : Range{b->s[sl].fp_begin, b->s[sl].fp_n}; }
-Range fs_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->fs_total} : Range{b->s[sl].fs_begin, b->s[sl].fs_n}; }
-Range fl_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->fl_total} : Range{b->s[sl].fl_begin, b->s[sl].fl_n}; }
 Range ew_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->ew_total} : Range{b->s[sl].ew_begin, b->s[sl].ew_n}; }
 Range sw_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->sw_total} : Range{b->s[sl].sw_begin, b->s[sl].sw_n}; }
 
@@ -396,25 +449,37 @@ void iradon_seq(acu_batch* b, const DevArray<RFTask>& rf, const DevArray<BPTask>
   check_launch();
 }
 
-// The gather projector always reads the cells; the LDS projector only where
-// a box can overflow (fpl_may_overflow).
-bool needs_cells(acu_batch* b, int sl) {
-  if (!b->fp_lds) return true;
-  if (sl >= 0) return b->s[sl].fpl_cells;
-  for (auto& s : b->s)
-    if (s.fpl_cells) return true;
-  return false;
-}
-
-void radon_seq(acu_batch* b, const DevArray<CellTask>& cl, const DevArray<FPTask>& fp,
-               const DevArray<FPLTask>& fl, int sl) {
+// radon of all slices (sl < 0) or of one slice with the task arrays of one
+// operand pair: cells where a slice's variant reads them, then one launch
+// per non-empty variant group.
+void radon_seq(acu_batch* b, const FPSet& set, int sl) {
   acu_ctx* c = b->ctx;
-  const Range r1 = cl_range(b, sl), r2 = fp_range(b, sl), r3 = fs_range(b, sl), r4 = fl_range(b, sl);
-  if (needs_cells(b, sl)) timed(c, K_CELLS, [&] { launch_cells(cl.d, cl.n, r1.base, r1.n, c->stream); });
-  if (b->fp_lds)
-    timed(c, K_FP, [&] { launch_forward_lds(fl.d, fl.n, r4.base, r4.n, b->fpl_al, c->stream); });
-  else
-    timed(c, K_FP, [&] { launch_forward(fp.d, fp.n, r2.base, r2.n, r3.base, r3.n, c->stream); });
+  static const int al_of[3] = {1, 2, 4};
+  if (sl < 0) {
+    if (b->cl_total > 0)
+      timed(c, K_CELLS, [&] { launch_cells(set.cl.d, set.cl.n, 0, b->cl_total, c->stream); });
+    for (int g = 0; g < 3; ++g)
+      if (b->fl_total[g] > 0)
+        timed(c, K_FP, [&] {
+          launch_forward_lds(set.fl[g].d, set.fl[g].n, 0, b->fl_total[g], al_of[g], c->stream);
+        });
+    if (b->fp_total > 0)
+      timed(c, K_FP, [&] { launch_forward(set.fp.d, set.fp.n, 0, b->fp_total, 0, b->fs_total, c->stream); });
+  } else {
+    const Slice& s = b->s[sl];
+    if (s.cl_n > 0)
+      timed(c, K_CELLS, [&] { launch_cells(set.cl.d, set.cl.n, s.cl_begin, s.cl_n, c->stream); });
+    if (s.fpv > 0) {
+      const int g = fl_group(s.fpv);
+      timed(c, K_FP, [&] {
+        launch_forward_lds(set.fl[g].d, set.fl[g].n, s.fl_begin, s.fl_n, s.fpv, c->stream);
+      });
+    } else {
+      timed(c, K_FP, [&] {
+        launch_forward(set.fp.d, set.fp.n, s.fp_begin, s.fp_n, s.fs_begin, s.fs_n, c->stream);
+      });
+    }
+  }
   check_launch();
 }
 
@@ -440,14 +505,14 @@ void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool reuse_ruk, bool 
   const int first = sl < 0 ? 0 : sl, count = sl < 0 ? nt : 1;  // finalize: one block per slice
   for (int ji = 0; ji < ninner; ++ji) {
     iradon_seq(b, b->rf_fk, b->bp_ifkt, sl);                       // :215
-    if (!(reuse_ruk && ji == 0)) radon_seq(b, b->cl_uk, b->fp_ruk, b->fl_ruk, sl);  // :228
+    if (!(reuse_ruk && ji == 0)) radon_seq(b, b->fp_uk, sl);  // :228
     iradon_seq(b, b->rf_ruk, b->bp_iukt, sl);                      // :229
     launch_tv_residual(tv, nt, ew.base, ew.n, c->stream);          // :207-249
     launch_tv_finalize(tv, first, count, 0, c->stream);
     check_launch();
     if (host_sync) (void)read_scalar(b, sl, 0);
     for (int j = 0; j < ncg; ++j) {                                // :252-303
-      radon_seq(b, b->cl_p, b->fp_rp, b->fl_rp, sl);
+      radon_seq(b, b->fp_p, sl);
       iradon_seq(b, b->rf_rp, b->bp_irpt, sl);
       launch_tv_apdot(tv, nt, ew.base, ew.n, j, c->stream);
       launch_tv_finalize(tv, first, count, 64 + j, c->stream);
@@ -465,7 +530,7 @@ void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool reuse_ruk, bool 
     check_launch();
     if (host_sync) (void)read_scalar(b, sl, 127);
   }
-  radon_seq(b, b->cl_uk, b->fp_ruk, b->fl_ruk, sl);                           // :339
+  radon_seq(b, b->fp_uk, sl);                                      // :339
   launch_tv_addback(tv, nt, sw.base, sw.n, c->stream);             // :341-346
   check_launch();
 }
@@ -497,7 +562,7 @@ void tvopt_run(acu_batch* b, float mu, float lambda, int nouter, int ninner, int
     for (int sl = 0; sl < (int)b->s.size(); ++sl)
       for (int jo = 0; jo < nouter; ++jo) outer_body(b, sl, ninner, ncg, false, true);
   } else {
-    if (nouter > 0) radon_seq(b, b->cl_uk, b->fp_ruk, b->fl_ruk, -1);  // ruk = radon(uk) for :228
+    if (nouter > 0) radon_seq(b, b->fp_uk, -1);  // ruk = radon(uk) for :228
     const bool graph = (mode == ACU_TV_GRAPH) && !c->timing;
     if (!graph) {
       for (int jo = 0; jo < nouter; ++jo) outer_body(b, -1, ninner, ncg, true, false);
@@ -547,17 +612,17 @@ Slice& slice_of(acu_batch* b, int sl) {
 void radon_dev(acu_batch* b, int sl, float* sino, const float* img) {
   Slice& s = slice_of(b, sl);
   acu_ctx* c = b->ctx;
-  if (needs_cells(b, sl)) {
+  if (s.cl_n > 0) {
     CellTask t = cell_task(s, img);
     t.block_begin = 0;
     CellTask* d = put_adhoc(b, t);
     timed(c, K_CELLS, [&] { launch_cells(d, 1, 0, s.cl_n, c->stream); });
   }
-  if (b->fp_lds) {
+  if (s.fpv > 0) {
     FPLTask t = fpl_task(s, img, sino);
     t.block_begin = 0;
     FPLTask* d = put_adhoc(b, t);
-    timed(c, K_FP, [&] { launch_forward_lds(d, 1, 0, s.fl_n, b->fpl_al, c->stream); });
+    timed(c, K_FP, [&] { launch_forward_lds(d, 1, 0, s.fl_n, s.fpv, c->stream); });
   } else {
     FPTask t = fp_task(s, sino);
     t.block_begin = 0;
@@ -586,6 +651,97 @@ void bp_dev(acu_batch* b, int sl, float* img, const float* filt) {
   BPTask* d = put_adhoc(b, t);
   timed(c, K_BP, [&] { launch_backproject(d, 1, 0, s.bp_n, c->stream); });
   check_launch();
+}
+
+// ---- autotuner: the forward-projection variant of every slice, by measurement ----
+uint64_t theta_hash(const std::vector<float>& th) {
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the angle bytes
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(th.data());
+  for (size_t i = 0; i < th.size() * sizeof(float); ++i) { h ^= p[i]; h *= 1099511628211ull; }
+  return h;
+}
+
+// Average ms of one radon of slice sl (FBP image -> ruk, contents
+// irrelevant to the timing) with variant v: cells when v reads them, then
+// the projector, launched back to back between two events.
+double time_variant(acu_batch* b, int sl, int v, int reps) {
+  Slice& s = b->s[sl];
+  acu_ctx* c = b->ctx;
+  const int saved = s.fpv;
+  s.fpv = v;
+  const bool cells = v == 0 || fpl_may_overflow(s.N, s.T, s.theta.data(), v);
+  DevArray<CellTask> cl;
+  DevArray<FPLTask> fl;
+  DevArray<FPTask> fp;
+  CellTask ct = cell_task(s, s.fbp);
+  ct.block_begin = 0;
+  cl.upload({ct});
+  int nfl = 0, nfp = 0, nfs = 0;
+  if (v > 0) {
+    FPLTask t = fpl_task(s, s.fbp, s.ruk);
+    t.block_begin = 0;
+    fl.upload({t});
+    nfl = fpl_blocks(s.N, s.T, v);
+  } else {
+    FPTask t = fp_task(s, s.ruk);
+    t.block_begin = 0;
+    t.sum_begin = 0;
+    fp.upload({t});
+    nfp = fp_ray_blocks(s.N) * s.T * fp_chunks(s.N, s.fp_clen);
+    nfs = fp_sum_blocks(s.N, s.T);
+  }
+  auto run = [&] {
+    if (cells) launch_cells(cl.d, 1, 0, cell_blocks(s.N), c->stream);
+    if (v > 0) launch_forward_lds(fl.d, 1, 0, nfl, v, c->stream);
+    else launch_forward(fp.d, 1, 0, nfp, 0, nfs, c->stream);
+  };
+  hipEvent_t e0 = c->ev(), e1 = c->ev();
+  run();  // warm-up
+  ACU_HIP_CHECK(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r) run();
+  ACU_HIP_CHECK(hipEventRecord(e1, c->stream));
+  check_launch();
+  ACU_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  ACU_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  c->pool.push_back(e0);
+  c->pool.push_back(e1);
+  cl.release(); fl.release(); fp.release();
+  s.fpv = saved;
+  return (double)ms / reps;
+}
+
+const char* variant_name(int v) { return v == 0 ? "gather" : v == 1 ? "lds1" : v == 2 ? "lds2" : "lds4"; }
+
+std::string autotune(acu_batch* b, int reps) {
+  require_theta(b);
+  b->ctx->sync();
+  static const int cand[4] = {1, 2, 4, 0};
+  std::map<std::tuple<int, int, uint64_t>, std::pair<int, std::vector<double>>> seen;
+  std::ostringstream js;
+  js << "{\"reps\":" << reps << ",\"slices\":[";
+  for (size_t i = 0; i < b->s.size(); ++i) {
+    Slice& s = b->s[i];
+    const auto key = std::make_tuple(s.N, s.T, theta_hash(s.theta));
+    auto it = seen.find(key);
+    if (it == seen.end()) {
+      std::vector<double> ms(4);
+      int best = 0;
+      for (int k = 0; k < 4; ++k) {
+        ms[k] = time_variant(b, (int)i, cand[k], reps);
+        if (ms[k] < ms[best]) best = k;
+      }
+      it = seen.emplace(key, std::make_pair(cand[best], ms)).first;
+    }
+    s.fpv = it->second.first;
+    js << (i ? "," : "") << "{\"slice\":" << i << ",\"N\":" << s.N << ",\"T\":" << s.T << ",\"ms\":{";
+    for (int k = 0; k < 4; ++k)
+      js << (k ? "," : "") << "\"" << variant_name(cand[k]) << "\":" << it->second.second[k];
+    js << "},\"choice\":\"" << variant_name(s.fpv) << "\"}";
+  }
+  js << "]}";
+  refresh_fp(b);
+  return js.str();
 }
 
 // ---- differentiation: exact adjoint and the gradients of
@@ -630,26 +786,12 @@ void ensure_grad(acu_batch* b) {
 
 // radon of every slice: imgs[i] -> sinos[i] (the batch's block ranges)
 void radon_many(acu_batch* b, const std::vector<const float*>& imgs, const std::vector<float*>& sinos) {
-  acu_ctx* c = b->ctx;
-  const int n = (int)b->s.size();
-  if (needs_cells(b, -1)) {
-    std::vector<CellTask> cl(n);
-    for (int i = 0; i < n; ++i) cl[i] = cell_task(b->s[i], imgs[i]);
-    b->cl_g.upload(cl);
-    timed(c, K_CELLS, [&] { launch_cells(b->cl_g.d, n, 0, b->cl_total, c->stream); });
-  }
-  if (b->fp_lds) {
-    std::vector<FPLTask> fl(n);
-    for (int i = 0; i < n; ++i) fl[i] = fpl_task(b->s[i], imgs[i], sinos[i]);
-    b->fl_g.upload(fl);
-    timed(c, K_FP, [&] { launch_forward_lds(b->fl_g.d, n, 0, b->fl_total, b->fpl_al, c->stream); });
-  } else {
-    std::vector<FPTask> fp(n);
-    for (int i = 0; i < n; ++i) fp[i] = fp_task(b->s[i], sinos[i]);
-    b->fp_g.upload(fp);
-    timed(c, K_FP, [&] { launch_forward(b->fp_g.d, n, 0, b->fp_total, 0, b->fs_total, c->stream); });
-  }
-  check_launch();
+  std::vector<const float*> im(imgs);
+  std::vector<float*> out(sinos);
+  std::map<const Slice*, int> idx;
+  for (size_t i = 0; i < b->s.size(); ++i) idx[&b->s[i]] = (int)i;
+  build_fpset(b, b->fp_g, [&](Slice& s) { return im[idx[&s]]; }, [&](Slice& s) { return out[idx[&s]]; });
+  radon_seq(b, b->fp_g, -1);
 }
 
 // x[i], shift[i] (nullable), gx[i], gs[i], hs[i] (nullable) are device
@@ -782,10 +924,14 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
   try {
     ACU_HIP_CHECK(hipSetDevice(ctx->device));
     b->s.resize(nslices);
-    int rf = 0, bp = 0, cl = 0, fp = 0, fs = 0, fl = 0, ew = 0, sw = 0;
-    // tuning: forward-projection path, ACUMEN_FP_PATH = lds (default) | gather
+    int rf = 0, bp = 0, ew = 0, sw = 0;
+    // forward-projection variant per slice: ACUMEN_FP_PATH = lds (default) |
+    // gather; ACUMEN_FPL_AL = 1 | 2 | 4 forces the LDS angles per lane;
+    // acu_batch_autotune replaces both by measurement
     const char* pv = std::getenv("ACUMEN_FP_PATH");
-    b->fp_lds = !(pv && std::strcmp(pv, "gather") == 0);
+    const bool gather = pv && std::strcmp(pv, "gather") == 0;
+    const char* av = std::getenv("ACUMEN_FPL_AL");
+    const int al_env = av ? std::atoi(av) : 0;
     // tuning: forward-projection sweep chunk length (steps), ACUMEN_FP_CHUNK
     const char* ev = std::getenv("ACUMEN_FP_CHUNK");
     const int fp_clen_env = ev ? std::atoi(ev) : 0;
@@ -797,21 +943,14 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       s.theta.assign(s.T, 0.0f);
       s.rf_begin = rf; s.rf_n = acu::rf_tiles_m(s.N) * acu::rf_tiles_n(s.T); rf += s.rf_n;
       s.bp_begin = bp; s.bp_n = s.ntx * s.nty; bp += s.bp_n;
-      s.cl_begin = cl; s.cl_n = acu::cell_blocks(s.N); cl += s.cl_n;
       s.fp_clen = fp_clen_env > 0 ? fp_clen_env : acu::FP_CHUNK_DEFAULT;
-      s.fp_begin = fp; s.fp_n = acu::fp_ray_blocks(s.N) * s.T * acu::fp_chunks(s.N, s.fp_clen); fp += s.fp_n;
-      s.fs_begin = fs; s.fs_n = acu::fp_sum_blocks(s.N, s.T); fs += s.fs_n;
-      b->fpl_al = std::min(b->fpl_al, acu::fpl_angles_per_lane(s.N, s.T));
+      s.fpv = gather ? 0
+                     : (al_env == 1 || al_env == 2 || al_env == 4) ? al_env
+                                                                   : acu::fpl_angles_per_lane(s.N, s.T);
       s.ew_begin = ew; s.ew_n = acu::ew_blocks((size_t)s.N * s.N); ew += s.ew_n;
       s.sw_begin = sw; s.sw_n = acu::ew_blocks((size_t)s.N * s.T); sw += s.sw_n;
     }
-    // one angle-block width for the whole batch (the kernel is specialised on it)
-    for (auto& s : b->s) {
-      s.fpl_al = b->fpl_al;
-      s.fl_begin = fl; s.fl_n = acu::fpl_blocks(s.N, s.T, s.fpl_al); fl += s.fl_n;
-    }
-    b->rf_total = rf; b->bp_total = bp; b->cl_total = cl; b->fp_total = fp; b->fs_total = fs;
-    b->fl_total = fl;
+    b->rf_total = rf; b->bp_total = bp;
     b->ew_total = ew; b->sw_total = sw;
     acu::Carver dry;
     acu::layout_ops(b.get(), dry);
@@ -826,6 +965,7 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
         acu::upload_theta(b.get(), i);
       }
     }
+    acu::assign_fp(b.get());
     acu::build_op_tasks(b.get());
   } catch (const Error& e) {
     return fail(ACU_ERR_HIP, e.msg);
@@ -856,9 +996,8 @@ int acu_batch_set_theta(acu_batch* b, int slice, const float* theta_deg) {
   ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync();
           std::memcpy(b->s[slice].theta.data(), theta_deg, sizeof(float) * b->s[slice].T);
           acu::upload_theta(b, slice);
-          // whether the cells pass is launched depends on the angles: a
-          // captured outer iteration is rebuilt on the next tvopt
-          if (b->graph) { ACU_HIP_CHECK(hipGraphExecDestroy(b->graph)); b->graph = nullptr; });
+          // whether the cells pass is launched depends on the angles
+          acu::refresh_fp(b));
 }
 
 int acu_batch_set_sino(acu_batch* b, int slice, const float* f_host) {
@@ -989,6 +1128,30 @@ int acu_get_slice(acu_batch* b, const float* f_host, float* img_fbp_host) {
   if (rc) return rc;
   if ((rc = acu_batch_fbp(b))) return rc;
   return acu_batch_get_image(b, 0, ACU_IMG_FBP, img_fbp_host);
+}
+
+int acu_batch_autotune(acu_batch* b, int reps, char* report, int cap) {
+  if (!b || reps < 1 || (report && cap < 1)) return fail(ACU_ERR_ARG, "acu_batch_autotune: bad argument");
+  ACU_TRY(hipSetDevice(b->ctx->device);
+          const std::string js = acu::autotune(b, reps);
+          if (report) {
+            const size_t n = std::min(js.size(), (size_t)cap - 1);
+            std::memcpy(report, js.data(), n);
+            report[n] = '\0';
+          });
+}
+
+int acu_batch_fp_variant(const acu_batch* b, int slice) {
+  if (!b || slice < 0 || slice >= (int)b->s.size()) return -fail(ACU_ERR_ARG, "acu_batch_fp_variant: bad slice");
+  return b->s[slice].fpv;
+}
+
+int acu_batch_set_fp_variant(acu_batch* b, int slice, int variant) {
+  if (!b || slice < 0 || slice >= (int)b->s.size())
+    return fail(ACU_ERR_ARG, "acu_batch_set_fp_variant: bad slice");
+  if (variant != 0 && variant != 1 && variant != 2 && variant != 4)
+    return fail(ACU_ERR_ARG, "acu_batch_set_fp_variant: variant must be 0, 1, 2 or 4");
+  ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync(); b->s[slice].fpv = variant; acu::refresh_fp(b));
 }
 
 int acu_radon_adjoint_dev(acu_batch* b, int slice, float* img_dev, const float* sino_dev) {

@@ -94,3 +94,23 @@ def poisson_noise(sino: np.ndarray, I0: float, N: int, seed: int) -> np.ndarray:
     lam = I0 * np.exp(-N * sino.astype(np.float64))
     c = rng.poisson(lam).astype(np.float64)
     return (-np.log(np.maximum(c, 1.0) / I0) / N).astype(np.float32)
+
+
+MIXED_WIDTHS = (512, 768, 1024, 1536, 2048)
+
+
+def mixed_batch(n: int = 64, seed: int = 1002):
+    """Slices of uneven size for the batching workload: detector width from
+    MIXED_WIDTHS and angle count from U{360..1500}, both seeded; per slice
+    an ellipse count in 20..40, Poisson I0 ~ U[2e3, 2e4] and a phantom seed.
+    Returns a list of dicts (N, T, wpu, theta, count, I0, seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n):
+        width = int(rng.choice(MIXED_WIDTHS))
+        T = int(rng.integers(360, 1501))
+        N = padded_size(width)
+        out.append(dict(N=N, T=T, wpu=default_wpu(N), theta=angles(T),
+                        count=int(rng.integers(20, 41)), I0=float(rng.uniform(2e3, 2e4)),
+                        seed=seed * 1000 + i))
+    return out

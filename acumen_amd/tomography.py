@@ -158,6 +158,27 @@ class Batch:
         check(self._lib.acu_iradon(self.handle, _fptr(out), _fptr(s)))
         return out
 
+    # ---- tuning ----
+    def autotune(self, reps: int = 2) -> dict:
+        """Time every forward-projection variant for every distinct slice
+        shape and keep the fastest per slice; returns the report (timings in
+        ms per radon, and the choice)."""
+        import json
+        cap = 256 + 192 * len(self.shapes)
+        buf = ctypes.create_string_buffer(cap)
+        check(self._lib.acu_batch_autotune(self.handle, int(reps), buf, cap))
+        return json.loads(buf.value.decode())
+
+    def fp_variant(self, i: int) -> int:
+        """0 = global gathers; 1, 2, 4 = LDS boxes with that many angles per lane."""
+        v = self._lib.acu_batch_fp_variant(self.handle, i)
+        if v < 0:
+            check(-v)
+        return v
+
+    def set_fp_variant(self, i: int, variant: int) -> None:
+        check(self._lib.acu_batch_set_fp_variant(self.handle, i, int(variant)))
+
     # ---- differentiation (no counterpart in the reference; built on radon) ----
     def radon_adjoint(self, sino) -> np.ndarray:
         """Exact transpose of :meth:`radon` (slice 0): (T,N) sinogram to (N,N) image."""

@@ -101,6 +101,99 @@ def cpu_baseline(N, T, wpu, threads):
     return threads / per_slice, sample, wall
 
 
+def run_mixed(args) -> int:
+    """64 slices, widths from {512, 768, 1024, 1536, 2048} and U{360..1500}
+    angles (seeded), 20-40 ellipses, I0 ~ U[2e3, 2e4], TV as the headline:
+    the batched path on all slices in one batch against the plain per-slice
+    path on the same slices.  slices/s over the timed steps."""
+    import numpy as np
+
+    from acumen_amd import ACU_IMG_TV, ACU_TV_BATCHED, ACU_TV_GRAPH, ACU_TV_PLAIN, Batch, Context, _lib, synth
+
+    specs = synth.mixed_batch(64, 1002)
+    ctx = Context(0)
+    log(f"inputs: {len(specs)} mixed slices")
+    sinos = []
+    for sp in specs:
+        with Batch(ctx, [(sp["N"], sp["T"], sp["wpu"], sp["theta"])]) as p:
+            s = p.radon(synth.phantom(sp["N"], sp["count"], sp["seed"]))
+        sinos.append(synth.poisson_noise(s, sp["I0"], sp["N"], sp["seed"] + 500))
+    shapes = [(sp["N"], sp["T"], sp["wpu"], sp["theta"]) for sp in specs]
+    batch = Batch(ctx, shapes)
+    for i, f in enumerate(sinos):
+        batch.set_sino(i, f)
+    ctx.synchronize()
+    tune_s, choices = None, None
+    if not args.no_tune:
+        log("autotune: forward-projection variant per slice shape")
+        t = time.perf_counter()
+        rep = batch.autotune(reps=2)
+        tune_s = time.perf_counter() - t
+        choices = {}
+        for sl in rep["slices"]:
+            choices[sl["choice"]] = choices.get(sl["choice"], 0) + 1
+    for w in range(args.warmup):
+        log(f"warmup step {w + 1}/{args.warmup}")
+        batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_GRAPH)
+        ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_GRAPH)
+        ctx.synchronize()
+        log(f"step {k + 1}/{args.steps} done at {time.perf_counter() - t0:.2f} s")
+    elapsed = time.perf_counter() - t0
+    value = len(specs) * args.steps / elapsed
+    assert all(np.all(np.isfinite(batch.get_image(i, ACU_IMG_TV))) for i in (0, len(specs) - 1))
+    # kernel breakdown of one batched step (events; graph replay is off while timing)
+    ctx.set_timing(True)
+    batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_BATCHED)
+    ctx.synchronize()
+    kt = {name: ctx.kernel_time(kind) for name, kind in
+          (("ramp", _lib.KIND_RAMP), ("bp", _lib.KIND_BP), ("cells", _lib.KIND_CELLS),
+           ("fp", _lib.KIND_FP))}
+    ctx.set_timing(False)
+    plain_rate = None
+    if not args.no_plain:
+        log("plain per-slice path: all slices, one after another")
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_PLAIN)
+        ctx.synchronize()
+        plain_rate = len(specs) / (time.perf_counter() - t2)
+    batch.close()
+    ctx.close()
+    widths = {}
+    for sp in specs:
+        widths[sp["N"] - 1] = widths.get(sp["N"] - 1, 0) + 1
+    out = {
+        "metric": "TV-SB slices/s, 64 mixed-size slices in one batch, batched vs plain",
+        "value": round(value, 5),
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "split-Bregman TV, 64 slices, widths from {512,768,1024,1536,2048} and "
+                               "U{360..1500} angles (seed 1002), 20-40 ellipses, I0~U[2e3,2e4], "
+                               f"mu={MU}, lambda={LAM}, {args.nouter} outer x {NCG} CG, graph-replayed "
+                               "batched path",
+                   "slices_per_step": len(specs), "width_counts": widths,
+                   "total_N2T": float(sum(sp["N"] ** 2 * sp["T"] for sp in specs))},
+        "plain_slices_per_s": None if plain_rate is None else round(plain_rate, 5),
+        "batched_vs_plain": None if plain_rate is None else round(value / plain_rate, 3),
+        "kernel_ms_per_step": {k: round(v[0], 2) for k, v in kt.items()},
+        "autotune_s": None if tune_s is None else round(tune_s, 2),
+        "fp_variant_counts": choices,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +205,10 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--nouter", type=int, default=NOUTER,
                     help="outer iterations (default: the workload's 50; fewer only for profiling runs)")
+    ap.add_argument("--no-tune", action="store_true", help="mixed workload: skip the autotuner")
+    ap.add_argument("--workload", choices=("production", "mixed"), default="production",
+                    help="production: 2048^2 x 1500 slices (the headline); mixed: 64 slices of "
+                         "uneven width and angle count in one batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -121,6 +218,9 @@ def main() -> int:
     if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
         print("bench.py: multi-process launch is not supported (one GPU)", file=sys.stderr)
         return 2
+
+    if args.workload == "mixed":
+        return run_mixed(args)
 
     import numpy as np
 

@@ -136,6 +136,23 @@ int acu_get_slice(acu_batch *b, const float *f_host, float *img_fbp_host);
 int acu_tvopt(acu_batch *b, const float *f_host, float mu, float lambda, int nouter, int ninner,
               int ncg, float *img_tv_host, float *img_fbp_host, float *delta_host);
 
+/* ---- Tuning.  No counterpart in the reference (its kernels are generated
+ * by the OpenACC compiler). ----
+ *
+ * Forward-projection variant of a slice: 0 = gathers of bilinear cells
+ * from global memory; 1, 2, 4 = LDS-staged pixel boxes with 4, 8 or 16
+ * angles per workgroup (1, 2, 4 per lane).  A batch starts from
+ * ACUMEN_FP_PATH / ACUMEN_FPL_AL or a per-shape rule; acu_batch_autotune
+ * times every variant (reps launches each, after one warm-up) for every
+ * distinct slice shape (N, T, angles) and keeps the fastest for all later
+ * calls.  report (may be NULL) receives a JSON summary of the timings and
+ * choices, truncated to cap bytes including the terminating NUL.  The
+ * getter returns the variant, or -ACU_ERR_ARG for a bad slice; the setter
+ * lets a caller reapply a stored choice. */
+int acu_batch_autotune(acu_batch *b, int reps, char *report, int cap);
+int acu_batch_fp_variant(const acu_batch *b, int slice);
+int acu_batch_set_fp_variant(acu_batch *b, int slice, int variant);
+
 /* ---- Differentiation.  No counterpart in the reference: these build on
  * its projector (tv_hpc.F90:129-159) and the periodic differences of its
  * tvopt (tv_hpc.F90:219-221) to make reconstruction and centre-of-rotation
