@@ -25,6 +25,7 @@ ACU_TV_BATCHED = 1
 ACU_TV_GRAPH = 2
 
 KIND_RAMP, KIND_BP, KIND_CELLS, KIND_FP, KIND_ADJ = 0, 1, 2, 3, 4
+RAMP_FP64, RAMP_FP32, RAMP_BF16X3, RAMP_BF16X6, RAMP_FFT = 0, 1, 2, 3, 4
 
 # Every symbol the header declares, with (restype, argtypes).
 _c_int, _c_float, _vp = ctypes.c_int, ctypes.c_float, ctypes.c_void_p
@@ -57,6 +58,8 @@ SIGNATURES = {
     "acu_tvopt": (_c_int, [_vp, _fp, _c_float, _c_float, _c_int, _c_int, _c_int, _fp, _fp, _fp]),
     "acu_batch_autotune": (_c_int, [_vp, _c_int, ctypes.c_char_p, _c_int]),
     "acu_batch_fp_variant": (_c_int, [_vp, _c_int]),
+    "acu_batch_set_ramp_variant": (_c_int, [_vp, _c_int]),
+    "acu_batch_ramp_variant": (_c_int, [_vp]),
     "acu_batch_set_fp_variant": (_c_int, [_vp, _c_int, _c_int]),
     "acu_radon_adjoint_dev": (_c_int, [_vp, _c_int, _vp, _vp]),
     "acu_radon_adjoint": (_c_int, [_vp, _fp, _fp]),

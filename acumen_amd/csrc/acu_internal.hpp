@@ -152,7 +152,26 @@ struct CellTask {
 // Launchers (stream-ordered, no host synchronisation).  A launch covers the
 // global blocks [base, base+blocks) of a task array: base = 0 and all blocks
 // for a whole batch, or one slice's own range.
-void launch_ramp(const RFTask* d_tasks, int ntasks, int base, int tiles, hipStream_t s);
+// arithmetic of the ramp filter (filter.hip); RAMP_FFT is fft.hip
+enum RampVariant { RAMP_FP64 = 0, RAMP_FP32 = 1, RAMP_BF16X3 = 2, RAMP_BF16X6 = 3, RAMP_FFT = 4 };
+void launch_ramp(const RFTask* d_tasks, int ntasks, int base, int tiles, int variant, hipStream_t s);
+
+// ---- ramp filter by FFT convolution (fft.hip) ----
+constexpr int FFT_MAX_L = 8192;  // transform length limit (LDS: 8 B per point)
+struct FFTTask {
+  int N, T, L, lg;     // L = 2^lg = fft_len(N)
+  int block_begin;     // one block per pair of angle rows
+  int alias;           // L == 2N-2: outputs 0 and N-1 get the correction
+  float g_edge;        // tap at distance N-1
+  const float* G;      // L: spectrum of the wrapped kernel, times 1/L
+  const float2* tw;    // L/2: exp(-2 pi i k / L)
+  const float* sino;   // (T, N) in
+  float* out;          // (T, N) out
+};
+int fft_len(int N);
+bool fft_supported(int N);
+void fft_tables(int N, const float* taps, float* G, float2* tw);
+void launch_ramp_fft(const FFTTask* d_tasks, int ntasks, int base, int blocks, int maxL, hipStream_t s);
 void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, hipStream_t s);
 void launch_cells(const CellTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
 void launch_forward(const FPTask* d_tasks, int ntasks, int base, int blocks, int sum_base,

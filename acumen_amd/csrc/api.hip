@@ -172,8 +172,19 @@ struct Slice {
         *gx_gs = nullptr, *gx_hs = nullptr;
   double *gx_pd = nullptr, *gx_pt = nullptr, *gx_loss = nullptr;
   // global block ranges in the batched launches
+  float* fftG = nullptr;   // FFT ramp: spectrum of the wrapped kernel / L (fft_len(N))
+  float2* fftW = nullptr;  // FFT ramp: twiddles (fft_len(N) / 2)
+  int ff_begin = 0, ff_n = 0;
+  float ff_edge = 0.0f;    // FFT ramp: the tap at distance N-1 (h(2N-1), 0-based)
   int rf_begin = 0, rf_n = 0, bp_begin = 0, bp_n = 0, cl_begin = 0, cl_n = 0,
       fp_begin = 0, fp_n = 0, fs_begin = 0, fs_n = 0, fl_begin = 0, fl_n = 0, ew_begin = 0, ew_n = 0, sw_begin = 0, sw_n = 0;
+};
+
+// Ramp-filter task arrays of one operand pair: the Toeplitz GEMM and the FFT form.
+struct RFSet {
+  DevArray<RFTask> rf;
+  DevArray<FFTTask> ff;
+  void release() { rf.release(); ff.release(); }
 };
 
 // Task arrays of one radon operand pair, one array per variant group; a
@@ -200,11 +211,13 @@ struct acu_batch {
   void* ops_mem = nullptr;
   void* tv_mem = nullptr;
   int tv_delta_cap = 0;
+  int ff_total = 0, ff_maxL = 0;  // FFT ramp blocks and the largest transform
   int rf_total = 0, bp_total = 0, cl_total = 0, fp_total = 0, fs_total = 0, ew_total = 0,
       sw_total = 0;
   int fl_total[3] = {0, 0, 0};  // LDS projector blocks per angles-per-lane group (1, 2, 4)
+  int ramp_variant = acu::RAMP_FP64;  // arithmetic of the ramp filter
   // task sets for the fixed operand combinations of FBP / TV
-  acu::DevArray<acu::RFTask> rf_f, rf_fk, rf_ruk, rf_rp;
+  acu::RFSet rf_f, rf_fk, rf_ruk, rf_rp;  // ramp filter of f, fk, ruk, rp
   acu::DevArray<acu::BPTask> bp_fbp, bp_ifkt, bp_iukt, bp_irpt;
   acu::FPSet fp_uk, fp_p;  // radon(uk) -> ruk, radon(p) -> rp
   acu::DevArray<acu::TVTask> tv;
@@ -270,6 +283,10 @@ void layout_ops(acu_batch* b, Carver& cv) {
     s.cells = cv.take<float4>(P * P);
     s.cellsT = cv.take<float4>(P * P);
     s.fpart = cv.take<float>((size_t)fp_chunks(s.N, s.fp_clen) * NT);
+    if (fft_supported(s.N)) {
+      s.fftG = cv.take<float>((size_t)fft_len(s.N));
+      s.fftW = cv.take<float2>((size_t)fft_len(s.N) / 2);
+    }
   }
 }
 
@@ -297,6 +314,15 @@ void upload_tables(acu_batch* b, int i) {
     for (int x = 0; x < s.N; ++x)
       if (cm[(size_t)y * s.N + x] != 0.0f) act[(size_t)(y / BP_TY) * s.ntx + x / BP_TX] = 1;
   ACU_HIP_CHECK(hipMemcpy(s.h, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  s.ff_edge = h[2 * s.N - 1];
+  if (s.fftG) {
+    const int L = fft_len(s.N);
+    std::vector<float> G(L);
+    std::vector<float2> W(L / 2);
+    fft_tables(s.N, h.data(), G.data(), W.data());
+    ACU_HIP_CHECK(hipMemcpy(s.fftG, G.data(), G.size() * 4, hipMemcpyHostToDevice));
+    ACU_HIP_CHECK(hipMemcpy(s.fftW, W.data(), W.size() * sizeof(float2), hipMemcpyHostToDevice));
+  }
   ACU_HIP_CHECK(hipMemcpy(s.cmask, cm.data(), cm.size() * 4, hipMemcpyHostToDevice));
   ACU_HIP_CHECK(hipMemcpy(s.tact, act.data(), act.size(), hipMemcpyHostToDevice));
 }
@@ -313,6 +339,12 @@ void upload_theta(acu_batch* b, int i) {
 
 RFTask rf_task(const Slice& s, const float* in, float* out) {
   return RFTask{s.N, s.T, s.rf_begin, rf_tiles_m(s.N), rf_tiles_n(s.T), s.h, in, out};
+}
+FFTTask ff_task(const Slice& s, const float* in, float* out) {
+  const int L = fft_len(s.N);
+  int lg = 0;
+  while ((1 << lg) < L) ++lg;
+  return FFTTask{s.N, s.T, L, lg, s.ff_begin, L == 2 * s.N - 2 ? 1 : 0, s.ff_edge, s.fftG, s.fftW, in, out};
 }
 BPTask bp_task(const Slice& s, const float* in, float* out) {
   return BPTask{s.N, s.T, s.mid, s.bp_begin, s.ntx, s.nty, in, s.cmask, s.scbp, s.tact, out};
@@ -375,6 +407,18 @@ void build_fpset(acu_batch* b, FPSet& set, ImgF img, OutF out) {
 
 void build_tv_tasks(acu_batch* b);
 
+template <class InF, class OutF>
+void build_rfset(acu_batch* b, RFSet& set, InF in, OutF out) {
+  std::vector<RFTask> rf;
+  std::vector<FFTTask> ff;
+  for (auto& s : b->s) {
+    rf.push_back(rf_task(s, in(s), out(s)));
+    if (s.fftG) ff.push_back(ff_task(s, in(s), out(s)));
+  }
+  set.rf.upload(rf);
+  set.ff.upload(ff);
+}
+
 // After a change of variants or angles: new ranges, new task arrays, and a
 // captured outer iteration is rebuilt on the next tvopt.
 void refresh_fp(acu_batch* b) {
@@ -384,14 +428,14 @@ void refresh_fp(acu_batch* b) {
 }
 
 void build_op_tasks(acu_batch* b) {
-  build(b, b->rf_f, [](Slice& s) { return rf_task(s, s.f, s.sinoCS); });
+  build_rfset(b, b->rf_f, [](Slice& s) { return (const float*)s.f; }, [](Slice& s) { return s.sinoCS; });
   build(b, b->bp_fbp, [](Slice& s) { return bp_task(s, s.sinoCS, s.fbp); });
 }
 
 void build_tv_tasks(acu_batch* b) {
-  build(b, b->rf_fk, [](Slice& s) { return rf_task(s, s.fk, s.sinoCS); });
-  build(b, b->rf_ruk, [](Slice& s) { return rf_task(s, s.ruk, s.sinoCS); });
-  build(b, b->rf_rp, [](Slice& s) { return rf_task(s, s.rp, s.sinoCS); });
+  build_rfset(b, b->rf_fk, [](Slice& s) { return (const float*)s.fk; }, [](Slice& s) { return s.sinoCS; });
+  build_rfset(b, b->rf_ruk, [](Slice& s) { return (const float*)s.ruk; }, [](Slice& s) { return s.sinoCS; });
+  build_rfset(b, b->rf_rp, [](Slice& s) { return (const float*)s.rp; }, [](Slice& s) { return s.sinoCS; });
   build(b, b->bp_ifkt, [](Slice& s) { return bp_task(s, s.sinoCS, s.ifkt); });
   build(b, b->bp_iukt, [](Slice& s) { return bp_task(s, s.sinoCS, s.iukt); });
   build(b, b->bp_irpt, [](Slice& s) { return bp_task(s, s.sinoCS, s.irpt); });
@@ -441,10 +485,21 @@ Range bp_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->bp_total} : R
 Range ew_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->ew_total} : Range{b->s[sl].ew_begin, b->s[sl].ew_n}; }
 Range sw_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->sw_total} : Range{b->s[sl].sw_begin, b->s[sl].sw_n}; }
 
-void iradon_seq(acu_batch* b, const DevArray<RFTask>& rf, const DevArray<BPTask>& bp, int sl) {
+void ramp_seq(acu_batch* b, const RFSet& rf, int sl) {
   acu_ctx* c = b->ctx;
-  const Range r1 = rf_range(b, sl), r2 = bp_range(b, sl);
-  timed(c, K_RF, [&] { launch_ramp(rf.d, rf.n, r1.base, r1.n, c->stream); });
+  if (b->ramp_variant == RAMP_FFT) {
+    const Range r = sl < 0 ? Range{0, b->ff_total} : Range{b->s[sl].ff_begin, b->s[sl].ff_n};
+    timed(c, K_RF, [&] { launch_ramp_fft(rf.ff.d, rf.ff.n, r.base, r.n, b->ff_maxL, c->stream); });
+  } else {
+    const Range r = rf_range(b, sl);
+    timed(c, K_RF, [&] { launch_ramp(rf.rf.d, rf.rf.n, r.base, r.n, b->ramp_variant, c->stream); });
+  }
+}
+
+void iradon_seq(acu_batch* b, const RFSet& rf, const DevArray<BPTask>& bp, int sl) {
+  acu_ctx* c = b->ctx;
+  const Range r2 = bp_range(b, sl);
+  ramp_seq(b, rf, sl);
   timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, c->stream); });
   check_launch();
 }
@@ -636,10 +691,18 @@ void radon_dev(acu_batch* b, int sl, float* sino, const float* img) {
 void ramp_dev(acu_batch* b, int sl, float* out, const float* in) {
   Slice& s = slice_of(b, sl);
   acu_ctx* c = b->ctx;
+  if (b->ramp_variant == RAMP_FFT) {
+    FFTTask t = ff_task(s, in, out);
+    t.block_begin = 0;
+    FFTTask* d = put_adhoc(b, t);
+    timed(c, K_RF, [&] { launch_ramp_fft(d, 1, 0, s.ff_n, fft_len(s.N), c->stream); });
+    check_launch();
+    return;
+  }
   RFTask t = rf_task(s, in, out);
   t.tile_begin = 0;
   RFTask* d = put_adhoc(b, t);
-  timed(c, K_RF, [&] { launch_ramp(d, 1, 0, s.rf_n, c->stream); });
+  timed(c, K_RF, [&] { launch_ramp(d, 1, 0, s.rf_n, b->ramp_variant, c->stream); });
   check_launch();
 }
 
@@ -932,6 +995,12 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
     const bool gather = pv && std::strcmp(pv, "gather") == 0;
     const char* av = std::getenv("ACUMEN_FPL_AL");
     const int al_env = av ? std::atoi(av) : 0;
+    // ramp-filter arithmetic: ACUMEN_RAMP = fp64 (default) | fp32 | bf16x3 | bf16x6 | fft
+    if (const char* rv = std::getenv("ACUMEN_RAMP")) {
+      const std::string r(rv);
+      b->ramp_variant = r == "fp32" ? acu::RAMP_FP32 : r == "bf16x3" ? acu::RAMP_BF16X3
+                        : r == "bf16x6" ? acu::RAMP_BF16X6 : r == "fft" ? acu::RAMP_FFT : acu::RAMP_FP64;
+    }
     // tuning: forward-projection sweep chunk length (steps), ACUMEN_FP_CHUNK
     const char* ev = std::getenv("ACUMEN_FP_CHUNK");
     const int fp_clen_env = ev ? std::atoi(ev) : 0;
@@ -942,6 +1011,8 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       s.ntx = acu::bp_tiles_x(s.N); s.nty = acu::bp_tiles_y(s.N);
       s.theta.assign(s.T, 0.0f);
       s.rf_begin = rf; s.rf_n = acu::rf_tiles_m(s.N) * acu::rf_tiles_n(s.T); rf += s.rf_n;
+      s.ff_begin = b->ff_total; s.ff_n = (s.T + 1) / 2; b->ff_total += s.ff_n;
+      b->ff_maxL = std::max(b->ff_maxL, acu::fft_len(s.N));
       s.bp_begin = bp; s.bp_n = s.ntx * s.nty; bp += s.bp_n;
       s.fp_clen = fp_clen_env > 0 ? fp_clen_env : acu::FP_CHUNK_DEFAULT;
       s.fpv = gather ? 0
@@ -965,6 +1036,10 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
         acu::upload_theta(b.get(), i);
       }
     }
+    if (b->ramp_variant == acu::RAMP_FFT)
+      for (auto& s : b->s)
+        if (!s.fftG)
+          throw Error("ACUMEN_RAMP=fft: the FFT filter supports N <= " + std::to_string(acu::FFT_MAX_L / 2 + 1));
     acu::assign_fp(b.get());
     acu::build_op_tasks(b.get());
   } catch (const Error& e) {
@@ -1139,6 +1214,24 @@ int acu_batch_autotune(acu_batch* b, int reps, char* report, int cap) {
             std::memcpy(report, js.data(), n);
             report[n] = '\0';
           });
+}
+
+int acu_batch_set_ramp_variant(acu_batch* b, int variant) {
+  if (!b) return fail(ACU_ERR_ARG, "null batch");
+  if (variant < acu::RAMP_FP64 || variant > acu::RAMP_FFT)
+    return fail(ACU_ERR_ARG, "acu_batch_set_ramp_variant: unknown variant");
+  if (variant == acu::RAMP_FFT)
+    for (auto& s : b->s)
+      if (!s.fftG)
+        return fail(ACU_ERR_ARG, "acu_batch_set_ramp_variant: the FFT filter supports N <= " +
+                                     std::to_string(acu::FFT_MAX_L / 2 + 1));
+  ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync(); b->ramp_variant = variant;
+          if (b->graph) { ACU_HIP_CHECK(hipGraphExecDestroy(b->graph)); b->graph = nullptr; });
+}
+
+int acu_batch_ramp_variant(const acu_batch* b) {
+  if (!b) return -fail(ACU_ERR_ARG, "null batch");
+  return b->ramp_variant;
 }
 
 int acu_batch_fp_variant(const acu_batch* b, int slice) {

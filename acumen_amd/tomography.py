@@ -169,6 +169,17 @@ class Batch:
         check(self._lib.acu_batch_autotune(self.handle, int(reps), buf, cap))
         return json.loads(buf.value.decode())
 
+    def set_ramp_variant(self, variant: int) -> None:
+        """Ramp-filter arithmetic: _lib.RAMP_FP64 (default), RAMP_FP32, RAMP_BF16X3,
+        RAMP_BF16X6, RAMP_FFT."""
+        check(self._lib.acu_batch_set_ramp_variant(self.handle, int(variant)))
+
+    def ramp_variant(self) -> int:
+        v = self._lib.acu_batch_ramp_variant(self.handle)
+        if v < 0:
+            check(-v)
+        return v
+
     def fp_variant(self, i: int) -> int:
         """0 = global gathers; 1, 2, 4 = LDS boxes with that many angles per lane."""
         v = self._lib.acu_batch_fp_variant(self.handle, i)

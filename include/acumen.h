@@ -150,6 +150,22 @@ int acu_tvopt(acu_batch *b, const float *f_host, float mu, float lambda, int nou
  * getter returns the variant, or -ACU_ERR_ARG for a bad slice; the setter
  * lets a caller reapply a stored choice. */
 int acu_batch_autotune(acu_batch *b, int reps, char *report, int cap);
+
+/* Arithmetic of the ramp filter (tv_hpc.F90:93-105) for every slice of the
+ * batch: ACU_RAMP_FP64 (default: FP32 inputs, exact products, FP64 sums on
+ * the FP64 matrix cores), ACU_RAMP_FP32 (FP32 matrix cores), ACU_RAMP_BF16X3
+ * (operands split into bf16 hi + lo, three products, FP32 sums),
+ * ACU_RAMP_BF16X6 (hi + mid + lo, six products, FP32 sums), ACU_RAMP_FFT
+ * (FFT convolution in FP32 with the kernel's spectrum from FP64; N <= 4097).
+ * The initial value comes from ACUMEN_RAMP = fp64 | fp32 | bf16x3 | bf16x6 |
+ * fft.  The getter returns the variant or -ACU_ERR_ARG. */
+#define ACU_RAMP_FP64 0
+#define ACU_RAMP_FP32 1
+#define ACU_RAMP_BF16X3 2
+#define ACU_RAMP_BF16X6 3
+#define ACU_RAMP_FFT 4
+int acu_batch_set_ramp_variant(acu_batch *b, int variant);
+int acu_batch_ramp_variant(const acu_batch *b);
 int acu_batch_fp_variant(const acu_batch *b, int slice);
 int acu_batch_set_fp_variant(acu_batch *b, int slice, int variant);
 
