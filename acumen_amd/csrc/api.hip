@@ -776,6 +776,35 @@ double time_variant(acu_batch* b, int sl, int v, int reps) {
 
 const char* variant_name(int v) { return v == 0 ? "gather" : v == 1 ? "lds1" : v == 2 ? "lds2" : "lds4"; }
 
+const char* ramp_name(int v) {
+  return v == RAMP_FP64 ? "fp64" : v == RAMP_FP32 ? "fp32" : v == RAMP_BF16X3 ? "bf16x3"
+         : v == RAMP_BF16X6 ? "bf16x6" : "fft";
+}
+
+// Average ms of one ramp filter of every slice (f -> sinoCS) with variant v.
+double time_ramp(acu_batch* b, int v, int reps) {
+  acu_ctx* c = b->ctx;
+  const int saved = b->ramp_variant;
+  b->ramp_variant = v;
+  auto run = [&] {
+    if (v == RAMP_FFT) launch_ramp_fft(b->rf_f.ff.d, b->rf_f.ff.n, 0, b->ff_total, b->ff_maxL, c->stream);
+    else launch_ramp(b->rf_f.rf.d, b->rf_f.rf.n, 0, b->rf_total, v, c->stream);
+  };
+  hipEvent_t e0 = c->ev(), e1 = c->ev();
+  run();
+  ACU_HIP_CHECK(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r) run();
+  ACU_HIP_CHECK(hipEventRecord(e1, c->stream));
+  check_launch();
+  ACU_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  ACU_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  c->pool.push_back(e0);
+  c->pool.push_back(e1);
+  b->ramp_variant = saved;
+  return (double)ms / reps;
+}
+
 std::string autotune(acu_batch* b, int reps) {
   require_theta(b);
   b->ctx->sync();
@@ -802,7 +831,28 @@ std::string autotune(acu_batch* b, int reps) {
       js << (k ? "," : "") << "\"" << variant_name(cand[k]) << "\":" << it->second.second[k];
     js << "},\"choice\":\"" << variant_name(s.fpv) << "\"}";
   }
-  js << "]}";
+  js << "]";
+  // ramp filter: the fastest arithmetic among those that meet the FBP / TV
+  // accuracy bar (bf16x3, with ~16 bits per operand, does not)
+  {
+    static const int rc[4] = {RAMP_FP64, RAMP_FP32, RAMP_BF16X6, RAMP_FFT};
+    bool fft_ok = true;
+    for (auto& s : b->s) fft_ok = fft_ok && s.fftG != nullptr;
+    int best = -1;
+    double best_ms = 0.0;
+    js << ",\"ramp\":{\"ms\":{";
+    bool first = true;
+    for (int k = 0; k < 4; ++k) {
+      if (rc[k] == RAMP_FFT && !fft_ok) continue;
+      const double ms = time_ramp(b, rc[k], reps);
+      js << (first ? "" : ",") << "\"" << ramp_name(rc[k]) << "\":" << ms;
+      first = false;
+      if (best < 0 || ms < best_ms) { best = rc[k]; best_ms = ms; }
+    }
+    b->ramp_variant = best;
+    js << "},\"choice\":\"" << ramp_name(best) << "\"}";
+  }
+  js << "}";
   refresh_fp(b);
   return js.str();
 }

@@ -161,10 +161,11 @@ class Batch:
     # ---- tuning ----
     def autotune(self, reps: int = 2) -> dict:
         """Time every forward-projection variant for every distinct slice
-        shape and keep the fastest per slice; returns the report (timings in
-        ms per radon, and the choice)."""
+        shape and keep the fastest per slice, and the ramp-filter arithmetics
+        that meet the accuracy bar on the whole batch; returns the report
+        (ms per radon per slice and per batch ramp filter, and the choices)."""
         import json
-        cap = 256 + 192 * len(self.shapes)
+        cap = 1024 + 192 * len(self.shapes)
         buf = ctypes.create_string_buffer(cap)
         check(self._lib.acu_batch_autotune(self.handle, int(reps), buf, cap))
         return json.loads(buf.value.decode())

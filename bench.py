@@ -132,6 +132,7 @@ def run_mixed(args) -> int:
         choices = {}
         for sl in rep["slices"]:
             choices[sl["choice"]] = choices.get(sl["choice"], 0) + 1
+        choices["ramp"] = rep["ramp"]["choice"]
     for w in range(args.warmup):
         log(f"warmup step {w + 1}/{args.warmup}")
         batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_GRAPH)
@@ -205,7 +206,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--nouter", type=int, default=NOUTER,
                     help="outer iterations (default: the workload's 50; fewer only for profiling runs)")
-    ap.add_argument("--no-tune", action="store_true", help="mixed workload: skip the autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
     ap.add_argument("--workload", choices=("production", "mixed"), default="production",
                     help="production: 2048^2 x 1500 slices (the headline); mixed: 64 slices of "
                          "uneven width and angle count in one batch")
@@ -236,6 +237,15 @@ def main() -> int:
     for i, f in enumerate(sinos):
         batch.set_sino(i, f)
     ctx.synchronize()
+    tuning = None
+    if not args.no_tune:
+        log("autotune: projector variant per slice shape, ramp-filter arithmetic")
+        t = time.perf_counter()
+        rep = batch.autotune(reps=2)
+        tuning = {"seconds": round(time.perf_counter() - t, 2),
+                  "fp_variant": rep["slices"][0]["choice"], "ramp": rep["ramp"]["choice"],
+                  "ramp_ms_per_batch": {k: round(v, 4) for k, v in rep["ramp"]["ms"].items()},
+                  "fp_ms_per_slice": {k: round(v, 4) for k, v in rep["slices"][0]["ms"].items()}}
 
     for w in range(args.warmup):
         log(f"warmup step {w + 1}/{args.warmup}")
@@ -275,6 +285,8 @@ def main() -> int:
     if not args.no_plain:
         log("plain per-slice path: 1 slice")
         with Batch(ctx, [(N, T, wpu, th)]) as one:
+            one.set_ramp_variant(batch.ramp_variant())
+            one.set_fp_variant(0, batch.fp_variant(0))
             one.set_sino(0, sinos[0])
             ctx.synchronize()
             t2 = time.perf_counter()
@@ -333,6 +345,7 @@ def main() -> int:
         "kernel_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "autotune": tuning,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -145,7 +145,9 @@ int acu_tvopt(acu_batch *b, const float *f_host, float mu, float lambda, int nou
  * ACUMEN_FP_PATH / ACUMEN_FPL_AL or a per-shape rule; acu_batch_autotune
  * times every variant (reps launches each, after one warm-up) for every
  * distinct slice shape (N, T, angles) and keeps the fastest for all later
- * calls.  report (may be NULL) receives a JSON summary of the timings and
+ * calls.  It also times the ramp-filter arithmetics that meet the FBP and
+ * TV accuracy bar (ACU_RAMP_FP64, _FP32, _BF16X6, _FFT; not _BF16X3) on the
+ * whole batch and keeps the fastest.  report (may be NULL) receives a JSON summary of the timings and
  * choices, truncated to cap bytes including the terminating NUL.  The
  * getter returns the variant, or -ACU_ERR_ARG for a bad slice; the setter
  * lets a caller reapply a stored choice. */

@@ -5,7 +5,7 @@ keeps results within the parity bar."""
 import numpy as np
 import pytest
 
-from acumen_amd import ACU_IMG_TV, ACU_TV_GRAPH, ACU_TV_PLAIN, AcumenError, Batch, synth
+from acumen_amd import ACU_IMG_TV, ACU_TV_GRAPH, ACU_TV_PLAIN, AcumenError, Batch, _lib, synth
 from helpers import disk_phantom, rel
 from oracle import ref
 
@@ -80,6 +80,13 @@ def test_autotune_report_and_results(ctx):
             assert names[b.fp_variant(i)] == s["choice"]
         # identical shapes are timed once and get the same choice
         assert rep["slices"][0]["ms"] == rep["slices"][2]["ms"]
+        # ramp filter: fastest among the arithmetics that meet the bar
+        rmp = rep["ramp"]
+        assert set(rmp["ms"]) == {"fp64", "fp32", "bf16x6", "fft"}
+        assert rmp["choice"] == min(rmp["ms"], key=rmp["ms"].get)
+        rnames = {_lib.RAMP_FP64: "fp64", _lib.RAMP_FP32: "fp32", _lib.RAMP_BF16X6: "bf16x6",
+                  _lib.RAMP_FFT: "fft"}
+        assert rnames[b.ramp_variant()] == rmp["choice"]
         b.tvopt(0.05, 0.1, 2, 1, 3, ACU_TV_GRAPH)
         for i, (N, T, w, th, f) in enumerate(items):
             tv64, _, _ = ref.tvopt(f, th, w, 0.05, 0.1, 2, 1, 3, "f64")
