@@ -1,5 +1,8 @@
 """acumen_amd -- MI355X (gfx950) reconstruction path of Acumen's neutron CT:
-radon / iradon (ramp filter + backprojection) / FBP / split-Bregman TV.
+radon / iradon (ramp filter + backprojection) / FBP / split-Bregman TV, with
+batching of uneven slices, an autotuner, differentiation (exact adjoint,
+gradients w.r.t. image and per-angle shifts; ``registration``) and volume
+streaming (``volume``).
 
 The compute lives in libacumen_hip.so (hand-written HIP kernels behind the C
 ABI of include/acumen.h); this package is its host-side front end.

@@ -69,6 +69,9 @@ SIGNATURES = {
     "acu_batch_loss_grad_dev": (_c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _c_float, _c_float,
                                          ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                          ctypes.POINTER(ctypes.c_double)]),
+    "acu_volume_reconstruct": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _fp, _fp, _fp, _fp, _c_float,
+                                        _c_float, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                        ctypes.POINTER(ctypes.c_double)]),
     "acu_ctx_set_timing": (_c_int, [_vp, _c_int]),
     "acu_ctx_kernel_time": (_c_int, [_vp, _c_int, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_longlong)]),

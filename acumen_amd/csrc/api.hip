@@ -5,6 +5,7 @@
 #include "acu_grad.hpp"
 #include "acumen.h"
 
+#include <chrono>
 #include <map>
 #include <sstream>
 #include <tuple>
@@ -1385,6 +1386,122 @@ int acu_batch_loss_grad(acu_batch* b, const float* const* x_host, const float* c
                                            c->stream));
           }
           c->sync());
+}
+
+namespace {
+// One of the two lanes of the volume pipeline: its own context (stream),
+// slice batches (full size and, lazily, the tail) and pinned staging.
+struct Lane {
+  acu_ctx* ctx = nullptr;
+  acu_batch* full = nullptr;
+  acu_batch* tail = nullptr;
+  float *sin_pin = nullptr, *fbp_pin = nullptr, *tv_pin = nullptr;
+  int group = -1, count = 0;
+  ~Lane() {
+    acu_batch_destroy(tail);
+    acu_batch_destroy(full);
+    acu_ctx_destroy(ctx);
+    if (sin_pin) hipHostFree(sin_pin);
+    if (fbp_pin) hipHostFree(fbp_pin);
+    if (tv_pin) hipHostFree(tv_pin);
+  }
+};
+
+void check_rc(int rc) {
+  if (rc != ACU_OK) throw Error(acu_last_error());
+}
+
+acu_batch* make_volume_batch(acu_ctx* c, int n, int N, int T, int wpu, const float* theta, const acu_batch* like) {
+  std::vector<int> Ns(n, N), Ts(n, T), ws(n, wpu);
+  std::vector<const float*> th(n, theta);
+  acu_batch* b = nullptr;
+  check_rc(acu_batch_create(c, n, Ns.data(), Ts.data(), ws.data(), th.data(), &b));
+  if (like) {  // same shape: the tuned variants carry over
+    b->ramp_variant = like->ramp_variant;
+    for (auto& s : b->s) s.fpv = like->s[0].fpv;
+    acu::refresh_fp(b);
+  }
+  return b;
+}
+
+void volume_run(acu_ctx* ctx, int nslices, int N, int T, int wpu, const float* theta, const float* sino,
+                float* fbp, float* tv, float mu, float lambda, int nouter, int ninner, int ncg, int batch,
+                int tune, double* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int B = std::min(batch, nslices);
+  const int groups = (nslices + B - 1) / B;
+  const size_t NT = (size_t)N * T, NN = (size_t)N * N;
+  Lane lanes[2];
+  for (auto& L : lanes) {
+    check_rc(acu_ctx_create(ctx->device, &L.ctx));
+    L.full = make_volume_batch(L.ctx, B, N, T, wpu, theta, nullptr);
+    ACU_HIP_CHECK(hipHostMalloc((void**)&L.sin_pin, (size_t)B * NT * 4, hipHostMallocDefault));
+    if (fbp) ACU_HIP_CHECK(hipHostMalloc((void**)&L.fbp_pin, (size_t)B * NN * 4, hipHostMallocDefault));
+    if (tv) ACU_HIP_CHECK(hipHostMalloc((void**)&L.tv_pin, (size_t)B * NN * 4, hipHostMallocDefault));
+  }
+  if (tune) {
+    (void)acu::autotune(lanes[0].full, 2);
+    lanes[1].full->ramp_variant = lanes[0].full->ramp_variant;
+    for (auto& s : lanes[1].full->s) s.fpv = lanes[0].full->s[0].fpv;
+    acu::refresh_fp(lanes[1].full);
+  }
+  // results of a lane's last group: wait for its stream, then unstage
+  auto finish = [&](Lane& L) {
+    if (L.group < 0) return;
+    L.ctx->sync();
+    const size_t first = (size_t)L.group * B;
+    if (fbp) std::memcpy(fbp + first * NN, L.fbp_pin, (size_t)L.count * NN * 4);
+    if (tv) std::memcpy(tv + first * NN, L.tv_pin, (size_t)L.count * NN * 4);
+    L.group = -1;
+  };
+  for (int g = 0; g < groups; ++g) {
+    Lane& L = lanes[g % 2];
+    finish(L);
+    const int n = std::min(B, nslices - g * B);
+    acu_batch* bt = L.full;
+    if (n < B) {
+      if (!L.tail) L.tail = make_volume_batch(L.ctx, n, N, T, wpu, theta, L.full);
+      bt = L.tail;
+    }
+    std::memcpy(L.sin_pin, sino + (size_t)g * B * NT, (size_t)n * NT * 4);
+    hipStream_t st = L.ctx->stream;
+    for (int i = 0; i < n; ++i)
+      ACU_HIP_CHECK(hipMemcpyAsync(bt->s[i].f, L.sin_pin + (size_t)i * NT, NT * 4, hipMemcpyHostToDevice, st));
+    if (tv) {
+      acu::tvopt_run(bt, mu, lambda, nouter, ninner, ncg, ACU_TV_GRAPH);
+    } else {
+      acu::fbp_all(bt, -1);
+      for (auto& s : bt->s) s.fbp_valid = true;
+    }
+    for (int i = 0; i < n; ++i) {
+      if (fbp)
+        ACU_HIP_CHECK(hipMemcpyAsync(L.fbp_pin + (size_t)i * NN, bt->s[i].fbp, NN * 4, hipMemcpyDeviceToHost, st));
+      if (tv)
+        ACU_HIP_CHECK(hipMemcpyAsync(L.tv_pin + (size_t)i * NN, bt->s[i].uk, NN * 4, hipMemcpyDeviceToHost, st));
+    }
+    L.group = g;
+    L.count = n;
+  }
+  finish(lanes[0]);
+  finish(lanes[1]);
+  if (stats) {
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats[0] = ms;
+    stats[1] = nslices / (ms * 1e-3);
+  }
+}
+}  // namespace
+
+int acu_volume_reconstruct(acu_ctx* ctx, int nslices, int nimage, int ntheta, int wpu, const float* theta_deg,
+                           const float* sino_host, float* fbp_host, float* tv_host, float mu, float lambda,
+                           int nouter, int ninner, int ncg, int batch, int tune, double* stats) {
+  if (!ctx || nslices < 1 || !theta_deg || !sino_host || batch < 1 || (!fbp_host && !tv_host))
+    return fail(ACU_ERR_ARG, "acu_volume_reconstruct: bad argument");
+  if (tv_host && (nouter < 1 || lambda == 0.0f))
+    return fail(ACU_ERR_ARG, "acu_volume_reconstruct: TV needs nouter >= 1 and lambda != 0");
+  ACU_TRY(hipSetDevice(ctx->device);
+          volume_run(ctx, nslices, nimage, ntheta, wpu, theta_deg, sino_host, fbp_host, tv_host, mu, lambda,
+                     nouter, ninner, ncg, batch, tune, stats));
 }
 
 int acu_tvopt(acu_batch* b, const float* f_host, float mu, float lambda, int nouter, int ninner,

@@ -206,6 +206,26 @@ int acu_batch_loss_grad_dev(acu_batch *b, const float *const *x_dev, const float
                             float lambda, float eps, float *const *grad_x_dev,
                             float *const *grad_s_dev, float *const *curv_s_dev, double *loss);
 
+/* ---- Volume streaming.  Replaces the reference drivers' slice loop over
+ * MPI ranks (tv_hpc.F90:583-631, fbp_hpc.F90:340-403) for a volume whose
+ * slices share one shape. ----
+ *
+ * nslices sinograms (sino_host: nslices * T * N floats, slice after slice,
+ * each (T, N) detector fastest as in sinograms{js}.dat) go through the GPU
+ * in batches of `batch` slices on two lanes, each with its own HIP stream
+ * and pinned staging buffers, so that one lane's host staging and PCIe
+ * transfers overlap the other lane's kernels.  fbp_host (nslices * N * N,
+ * may be NULL) receives get_slice (fbp_hpc.F90:129-150); tv_host (may be
+ * NULL) receives tvopt's TV image (tv_hpc.F90:162-363) with mu, lambda,
+ * nouter >= 1, ninner, ncg.  tune = 1 runs acu_batch_autotune on the first
+ * batch and uses its choices throughout.  stats (may be NULL) receives
+ * [0] wall milliseconds of the call, [1] slices per second, PCIe and host
+ * staging included.  Synchronous; uses device memory for 2 batches. */
+int acu_volume_reconstruct(acu_ctx *ctx, int nslices, int nimage, int ntheta, int wpu,
+                           const float *theta_deg, const float *sino_host, float *fbp_host,
+                           float *tv_host, float mu, float lambda, int nouter, int ninner, int ncg,
+                           int batch, int tune, double *stats);
+
 /* Kernel timing: when enabled, every ramp-filter, backprojection, cells and
  * forward-projection launch is bracketed by HIP events on the context's
  * stream (graph mode is then run as ACU_TV_BATCHED).  acu_ctx_kernel_time
