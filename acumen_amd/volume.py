@@ -23,8 +23,9 @@ def reconstruct(ctx, sinos, theta, wpu: int, tv=None, fbp: bool = True, batch: i
     TV) of every slice of ``sinos`` (S, T, N) float32.
 
     Returns ``(fbp_images or None, tv_images or None, stats)`` with images
-    (S, N, N) and ``stats = {"ms": wall ms, "slices_per_s": ...}``, PCIe and
-    host staging included."""
+    (S, N, N) and ``stats`` = wall ``ms``, ``slices_per_s`` over the call,
+    ``setup_ms`` (lanes, pinned buffers, tuning) and ``stream_slices_per_s``
+    after set-up; PCIe and host staging included."""
     s = np.ascontiguousarray(sinos, np.float32)
     if s.ndim != 3:
         raise ValueError("sinos must be (slices, angles, detectors)")
@@ -37,7 +38,7 @@ def reconstruct(ctx, sinos, theta, wpu: int, tv=None, fbp: bool = True, batch: i
     p = dict(mu=0.0, lam=0.0, nouter=0, ninner=1, ncg=5)
     if tv is not None:
         p.update(tv)
-    stats = np.zeros(2, np.float64)
+    stats = np.zeros(4, np.float64)
     lib = _lib.load()
     check(lib.acu_volume_reconstruct(
         ctx.handle, S, N, T, int(wpu), th.ctypes.data_as(_F), s.ctypes.data_as(_F),
@@ -45,4 +46,5 @@ def reconstruct(ctx, sinos, theta, wpu: int, tv=None, fbp: bool = True, batch: i
         None if out_tv is None else out_tv.ctypes.data_as(_F),
         float(p["mu"]), float(p["lam"]), int(p["nouter"]), int(p["ninner"]), int(p["ncg"]),
         int(batch), 1 if tune else 0, stats.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
-    return out_fbp, out_tv, {"ms": float(stats[0]), "slices_per_s": float(stats[1])}
+    return out_fbp, out_tv, {"ms": float(stats[0]), "slices_per_s": float(stats[1]),
+                             "setup_ms": float(stats[2]), "stream_slices_per_s": float(stats[3])}

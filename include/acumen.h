@@ -218,9 +218,13 @@ int acu_batch_loss_grad_dev(acu_batch *b, const float *const *x_dev, const float
  * may be NULL) receives get_slice (fbp_hpc.F90:129-150); tv_host (may be
  * NULL) receives tvopt's TV image (tv_hpc.F90:162-363) with mu, lambda,
  * nouter >= 1, ninner, ncg.  tune = 1 runs acu_batch_autotune on the first
- * batch and uses its choices throughout.  stats (may be NULL) receives
- * [0] wall milliseconds of the call, [1] slices per second, PCIe and host
- * staging included.  Synchronous; uses device memory for 2 batches. */
+ * batch and uses its choices throughout.  stats (may be NULL, else 4
+ * doubles) receives [0] wall milliseconds of the call, [1] slices per
+ * second over the call, [2] milliseconds of set-up (lanes, pinned buffers,
+ * tuning), [3] slices per second of the streaming after set-up; PCIe and
+ * host staging are included in all.  Synchronous; uses device memory for 2
+ * batches.  ACUMEN_COPY_THREADS (default 8) sets the host threads of the
+ * staging copies. */
 int acu_volume_reconstruct(acu_ctx *ctx, int nslices, int nimage, int ntheta, int wpu,
                            const float *theta_deg, const float *sino_host, float *fbp_host,
                            float *tv_host, float mu, float lambda, int nouter, int ninner, int ncg,
