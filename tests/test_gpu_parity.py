@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from acumen_amd import ACU_IMG_FBP, ACU_IMG_TV, ACU_TV_BATCHED, ACU_TV_GRAPH, ACU_TV_PLAIN
-from acumen_amd import Batch, Tomography, synth
+from acumen_amd import Batch, Tomography, _lib, synth
 from helpers import HipBuffers, disk_phantom, interior_noise, rel
 from oracle import ref
 
@@ -158,6 +158,37 @@ def test_full_size_fbp_and_radon(ctx):
     assert rel(s, ref.radon(x, th, "f32")) < FBP_TOL
     assert rel(img, ref.iradon(s, th, wpu, "f32")) < FBP_TOL
     assert rel(img, ref.iradon(s, th, wpu, "f64")) < 2.5e-5
+
+
+def test_full_size_tv_against_fp64(ctx):
+    """The production slice (2049^2 x 1500, 40-ellipse phantom, Poisson
+    I0 = 5e3): split-Bregman TV, 1 outer x 2 CG iterations, against the FP64
+    oracle -- with the library defaults and with the autotuner's choices at
+    this shape (LDS projector with 4 angles per lane, FFT ramp filter)."""
+    N, T = 2049, 1500
+    say("full-size TV", N, T)
+    th = synth.angles(T)
+    wpu = synth.default_wpu(N)
+    x = synth.phantom(N, 40, 1001)
+    results = {}
+    with plan(ctx, N, T, th) as b:
+        f = synth.poisson_noise(b.radon(x), 5e3, N, 1101)
+        b.set_sino(0, f)
+        b.tvopt(0.05, 0.1, 1, 1, 2, ACU_TV_GRAPH)
+        results["default"] = b.get_image(0, ACU_IMG_TV)
+        b.set_fp_variant(0, 4)
+        b.set_ramp_variant(_lib.RAMP_FFT)
+        b.tvopt(0.05, 0.1, 1, 1, 2, ACU_TV_GRAPH)
+        results["tuned"] = b.get_image(0, ACU_IMG_TV)
+    ref.set_threads(16)
+    try:
+        tv64, _, _ = ref.tvopt(f, th, wpu, 0.05, 0.1, 1, 1, 2, "f64")
+    finally:
+        ref.set_threads(1)
+    for k, img in results.items():
+        e = rel(img, tv64)
+        print("full-size TV", k, "rel-L2 vs FP64 %.2e" % e, flush=True)
+        assert e < TV_TOL, (k, e)
 
 
 @pytest.mark.parametrize("N,T,nouter", [(65, 48, 2), (129, 96, 5)])
