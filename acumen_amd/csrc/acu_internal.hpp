@@ -81,6 +81,12 @@ constexpr int FPL_U = ACUMEN_FPL_U;          // steps per staged chunk
 constexpr int FPL_PITCH = ACUMEN_FPL_PITCH;  // LDS row pitch of the box (odd; pitch+1 <= 255 dwords)
 constexpr int FPL_CAP = ACUMEN_FPL_PITCH * ACUMEN_FPL_ROWS;  // floats of the LDS box
 static_assert(FPL_PITCH + 1 <= 255, "ds_read2 offsets");
+// Row pitches an angle block may use for its boxes (the capacity FPL_CAP is
+// fixed; rows = FPL_CAP / pitch); class FPL_PCLASS_DEFAULT is FPL_PITCH.
+constexpr int FPL_NPITCH = 8;
+constexpr int FPL_PITCHES[FPL_NPITCH] = {93, 97, 99, 101, 103, 105, 107, 109};
+constexpr int FPL_PCLASS_DEFAULT = 3;
+static_assert(FPL_PITCHES[FPL_PCLASS_DEFAULT] == FPL_PITCH, "default pitch class");
 
 // ---- ramp filter GEMM (filter.hip) ----
 constexpr int RF_BM = 64;   // detector rows per tile
@@ -130,6 +136,7 @@ struct FPLTask {
   const float4* cells;   // x-major cells of the same image (fallback for oversized boxes)
   const float2* sc;      // per-angle (sin, cos) for projection
   float* sino;           // N*T out
+  const uint8_t* pclass; // nab pitch classes (fpl_pitch_classes), or null: FPL_PITCH
 };
 
 struct RFTask {
@@ -192,6 +199,10 @@ int fpl_blocks(int N, int T, int al);
 // true when some chunk of the LDS projector can outgrow its box and read
 // the global bilinear cells (which then have to be built first)
 bool fpl_may_overflow(int N, int T, const float* theta_deg, int al);
+// Pitch class of every block of 4*al angles: the pitch with the fewest LDS
+// bank conflicts for the block's angle among those whose rows hold the
+// block's box (out: ceil(T / (4*al)) bytes)
+void fpl_pitch_classes(int N, int T, const float* theta_deg, int al, uint8_t* out);
 int cell_blocks(int N);
 
 }  // namespace acu

@@ -159,6 +159,7 @@ struct Slice {
   // (fp.hip), 1 / 2 / 4 = LDS boxes with that many angles per lane (fpl.hip)
   int fpv = 2;
   bool fpl_cells = true;  // the LDS projector may need the global cells of this slice
+  uint8_t* pcls[3] = {nullptr, nullptr, nullptr};  // pitch classes of the angle blocks, al = 1, 2, 4
   // TV arena
   float *uk = nullptr, *ukp = nullptr, *dkx = nullptr, *dky = nullptr, *bkx = nullptr,
         *bky = nullptr, *ifkt = nullptr, *iukt = nullptr, *r = nullptr, *p = nullptr,
@@ -285,6 +286,7 @@ void layout_ops(acu_batch* b, Carver& cv) {
     s.cells = cv.take<float4>(P * P);
     s.cellsT = cv.take<float4>(P * P);
     s.fpart = cv.take<float>((size_t)fp_chunks(s.N, s.fp_clen) * NT);
+    for (int g = 0; g < 3; ++g) s.pcls[g] = cv.take<uint8_t>((size_t)(s.T + (4 << g) - 1) / (4 << g));
     if (fft_supported(s.N)) {
       s.fftG = cv.take<float>((size_t)fft_len(s.N));
       s.fftW = cv.take<float2>((size_t)fft_len(s.N) / 2);
@@ -336,6 +338,12 @@ void upload_theta(acu_batch* b, int i) {
   make_angles(s.T, s.theta.data(), false, fp.data());
   ACU_HIP_CHECK(hipMemcpy(s.scbp, bp.data(), bp.size() * sizeof(float2), hipMemcpyHostToDevice));
   ACU_HIP_CHECK(hipMemcpy(s.scfp, fp.data(), fp.size() * sizeof(float2), hipMemcpyHostToDevice));
+  for (int g = 0; g < 3; ++g) {
+    const int al = 1 << g;
+    std::vector<uint8_t> pc((size_t)(s.T + 4 * al - 1) / (4 * al));
+    fpl_pitch_classes(s.N, s.T, s.theta.data(), al, pc.data());
+    ACU_HIP_CHECK(hipMemcpy(s.pcls[g], pc.data(), pc.size(), hipMemcpyHostToDevice));
+  }
   s.theta_set = true;
 }
 
@@ -352,8 +360,12 @@ BPTask bp_task(const Slice& s, const float* in, float* out) {
   return BPTask{s.N, s.T, s.mid, s.bp_begin, s.ntx, s.nty, in, s.cmask, s.scbp, s.tact, out};
 }
 FPLTask fpl_task(const Slice& s, const float* img, float* out) {
+  static const bool fixed_pitch = [] {
+    const char* e = std::getenv("ACUMEN_FPL_PITCH");
+    return e && std::strcmp(e, "fixed") == 0;
+  }();
   return FPLTask{s.N, s.T, s.mid, s.fl_begin, s.fpv, (s.T + 4 * s.fpv - 1) / (4 * s.fpv),
-                 img, s.cells, s.scfp, out};
+                 img, s.cells, s.scfp, out, fixed_pitch ? nullptr : s.pcls[fl_group(s.fpv)]};
 }
 CellTask cell_task(const Slice& s, const float* img) {
   return CellTask{s.N, s.cl_begin, img, s.cells, s.cellsT};

@@ -72,18 +72,13 @@ __device__ __forceinline__ void urange(float a, float b, float lo, float hi, flo
   }
 }
 
-template <int AL>
-__global__ __launch_bounds__(FPL_THREADS) void fpl_kernel(const FPLTask* __restrict__ tasks, int ntasks,
-                                                          int base, int nblocks) {
+// One workgroup's work; PITCH is the LDS row pitch of its boxes (odd, so
+// that the taps of 64 neighbouring rays spread over the banks; which odd
+// value spreads them best depends on the angle, see fpl_pitch_classes).
+template <int AL, int PITCH>
+__device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __restrict__ box, float (*red)[4]) {
 #pragma clang fp contract(off)
-  constexpr int PITCH = FPL_PITCH;
-  __shared__ float box[FPL_CAP];
-  __shared__ float red[4][4];  // per-wave min x, min y, max x, max y (or step range)
-
-  const int lbk = xcd_remap(blockIdx.x, gridDim.x);
-  if (lbk >= nblocks) return;
-  const int gb = base + lbk;
-  const FPLTask& tk = tasks[find_task(tasks, ntasks, gb)];
+  static_assert(PITCH + 1 <= 255, "ds_read2 offsets");
   const int N = tk.N, T = tk.T;
   const float m = tk.mid;
   const int local = gb - tk.block_begin;
@@ -269,6 +264,28 @@ __global__ __launch_bounds__(FPL_THREADS) void fpl_kernel(const FPLTask* __restr
     if (act[j]) tk.sino[(size_t)ang[j] * N + ray] = acc[j] / (float)N;
 }
 
+template <int AL>
+__global__ __launch_bounds__(FPL_THREADS) void fpl_kernel(const FPLTask* __restrict__ tasks, int ntasks,
+                                                          int base, int nblocks) {
+  __shared__ float box[FPL_CAP];
+  __shared__ float red[4][4];  // per-wave min x, min y, max x, max y (or step range)
+  const int lbk = xcd_remap(blockIdx.x, gridDim.x);
+  if (lbk >= nblocks) return;
+  const int gb = base + lbk;
+  const FPLTask& tk = tasks[find_task(tasks, ntasks, gb)];
+  const int ab = (gb - tk.block_begin) % tk.nab;
+  switch (tk.pclass ? (int)tk.pclass[ab] : FPL_PCLASS_DEFAULT) {
+    case 0: fpl_body<AL, FPL_PITCHES[0]>(tk, gb, box, red); break;
+    case 1: fpl_body<AL, FPL_PITCHES[1]>(tk, gb, box, red); break;
+    case 2: fpl_body<AL, FPL_PITCHES[2]>(tk, gb, box, red); break;
+    case 4: fpl_body<AL, FPL_PITCHES[4]>(tk, gb, box, red); break;
+    case 5: fpl_body<AL, FPL_PITCHES[5]>(tk, gb, box, red); break;
+    case 6: fpl_body<AL, FPL_PITCHES[6]>(tk, gb, box, red); break;
+    case 7: fpl_body<AL, FPL_PITCHES[7]>(tk, gb, box, red); break;
+    default: fpl_body<AL, FPL_PITCHES[FPL_PCLASS_DEFAULT]>(tk, gb, box, red); break;
+  }
+}
+
 }  // namespace
 
 int fpl_angles_per_lane(int N, int T) {
@@ -282,6 +299,73 @@ int fpl_angles_per_lane(int N, int T) {
   while (al < AMAX && (4.0 * (2 * al) - 1.0) * dth * r <= room) al *= 2;
   return al;
 }
+// Bank cost of the two ds_read2 of one sample for 64 neighbouring rays
+// (one per lane) at projector angle (s, c), summed over a few sub-pixel
+// offsets: per half-wave, the largest number of distinct dwords one of the
+// 32 banks serves.
+static int pitch_cost(int P, double s, double c) {
+  static const double off[4][2] = {{0.13, 0.71}, {0.57, 0.29}, {0.91, 0.47}, {0.35, 0.05}};
+  int cost = 0;
+  for (const auto& o : off) {
+    int addr[64];
+    for (int k = 0; k < 64; ++k) {
+      const double x = 50.0 + o[0] - s * k, y = 50.0 + o[1] + c * k;
+      addr[k] = (int)std::floor(y) * P + (int)std::floor(x);
+    }
+    for (int rd = 0; rd < 2; ++rd)
+      for (int h = 0; h < 2; ++h) {
+        int d[64], nd = 0;
+        for (int k = 32 * h; k < 32 * h + 32; ++k)
+          for (int e = 0; e < 2; ++e) {
+            const int a = addr[k] + rd * P + e;
+            bool seen = false;
+            for (int q = 0; q < nd; ++q) seen = seen || d[q] == a;
+            if (!seen) d[nd++] = a;
+          }
+        int cnt[32] = {0}, mx = 0;
+        for (int q = 0; q < nd; ++q) mx = std::max(mx, ++cnt[((d[q] % 32) + 32) % 32]);
+        cost += mx;
+      }
+  }
+  return cost;
+}
+
+void fpl_pitch_classes(int N, int T, const float* theta_deg, int al, uint8_t* out) {
+  const double pi = 3.141592653589793;
+  const double r = 0.70711 * ((double)N + 2.0);
+  const int ab = 4 * al;
+  for (int a0 = 0, blk = 0; a0 < T; a0 += ab, ++blk) {
+    const int a1 = std::min(T, a0 + ab) - 1;
+    double lo = 1e300, hi = -1e300, ms = 0.0, mc = 0.0;
+    for (int a = a0; a <= a1; ++a) {
+      const double ct = pi * ((double)theta_deg[a] - 90.0) / 180.0;
+      lo = std::min(lo, ct);
+      hi = std::max(hi, ct);
+      ms = std::max(ms, std::fabs(std::sin(ct)));
+      mc = std::max(mc, std::fabs(std::cos(ct)));
+    }
+    // per-axis box of a chunk (as in fpl_may_overflow, per axis): a fixed
+    // footprint plus the drift of a sample across the block's angles, which
+    // grows with the distance from the centre (at most `drift`); a pitch
+    // tolerates the drift min(P - bx, rows - by) before a chunk overflows
+    const double drift = (hi - lo + 1e-6) * r;
+    const double bx = 63.0 * ms + (FPL_U - 1) * mc + 8.0, by = 63.0 * mc + (FPL_U - 1) * ms + 8.0;
+    auto tolerance = [&](int P) { return std::min((double)P - bx, (double)(FPL_CAP / P) - by); };
+    const double base_tol = std::min(tolerance(FPL_PITCHES[FPL_PCLASS_DEFAULT]), drift);
+    const double mid = pi * (0.5 * ((double)theta_deg[a0] + (double)theta_deg[a1]) - 90.0) / 180.0;
+    const double s = std::sin(mid), c = std::cos(mid);
+    int best = FPL_PCLASS_DEFAULT, best_cost = pitch_cost(FPL_PITCHES[FPL_PCLASS_DEFAULT], s, c);
+    for (int k = 0; k < FPL_NPITCH; ++k) {
+      const int P = FPL_PITCHES[k];
+      // no more overflowing chunks than the default pitch
+      if (k == FPL_PCLASS_DEFAULT || tolerance(P) < base_tol) continue;
+      const int cost = pitch_cost(P, s, c);
+      if (cost < best_cost) { best = k; best_cost = cost; }
+    }
+    out[blk] = (uint8_t)best;
+  }
+}
+
 bool fpl_may_overflow(int N, int T, const float* theta_deg, int al) {
   // The box of a chunk spans (per axis) at most the 64-ray x FPL_U-step
   // footprint of one angle, hypot(63, FPL_U - 1), plus the drift of a
