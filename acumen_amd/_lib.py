@@ -72,6 +72,13 @@ SIGNATURES = {
     "acu_volume_reconstruct": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _fp, _fp, _fp, _fp, _c_float,
                                         _c_float, _c_int, _c_int, _c_int, _c_int, _c_int,
                                         ctypes.POINTER(ctypes.c_double)]),
+    "acu_prep_median27_dev": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp]),
+    "acu_prep_correct_dev": (_c_int, [_vp, _c_int, _c_int, _c_float, _c_float, _vp, _c_int, _c_int, _c_int,
+                                      _vp]),
+    "acu_prep_mirror_error_dev": (_c_int, [_vp, _c_int, _c_int, _c_float, _c_float, _vp, _vp,
+                                           ctypes.POINTER(ctypes.c_double)]),
+    "acu_prep_find_center": (_c_int, [_vp, _c_int, _fp, _fp, _fp, _fp, ctypes.POINTER(_c_int)]),
+    "acu_prep_sinograms": (_c_int, [_vp, _c_int, _c_int, _fp, _c_float, _c_float, _fp]),
     "acu_ctx_set_timing": (_c_int, [_vp, _c_int]),
     "acu_ctx_kernel_time": (_c_int, [_vp, _c_int, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_longlong)]),

@@ -3,6 +3,7 @@
 // operator launches, and the split-Bregman TV orchestration.
 #include "acu_tv.hpp"
 #include "acu_grad.hpp"
+#include "acu_prep.hpp"
 #include "acumen.h"
 
 #include <chrono>
@@ -1540,6 +1541,169 @@ int acu_volume_reconstruct(acu_ctx* ctx, int nslices, int nimage, int ntheta, in
   ACU_TRY(hipSetDevice(ctx->device);
           volume_run(ctx, nslices, nimage, ntheta, wpu, theta_deg, sino_host, fbp_host, tv_host, mu, lambda,
                      nouter, ninner, ncg, batch, tune, stats));
+}
+
+// ---- projection -> sinogram preprocessing (projections_to_sinogram_hpc.F90) ----
+namespace {
+// device scratch of the mirror-error sums: block partials + the total
+struct PrepScratch {
+  double* d = nullptr;
+  size_t n = 0;
+  ~PrepScratch() { if (d) hipFree(d); }
+  double* get(size_t need) {
+    if (need > n) {
+      if (d) ACU_HIP_CHECK(hipFree(d));
+      d = nullptr;
+      ACU_HIP_CHECK(hipMalloc(&d, need * sizeof(double)));
+      n = need;
+    }
+    return d;
+  }
+};
+
+double mirror_error(acu_ctx* c, PrepScratch& sc, int n, int add_nx, float theta, float shift, const float* p0,
+                    const float* p180) {
+  const acu::PrepGeom g = acu::prep_geom(n, add_nx, theta, shift);
+  const int nb = acu::mirror_blocks(g);
+  double* d = sc.get((size_t)nb + 1);
+  acu::launch_mirror(g, p0, p180, d, d + nb, c->stream);
+  ACU_HIP_CHECK(hipGetLastError());
+  double e = 0.0;
+  ACU_HIP_CHECK(hipMemcpyAsync(&e, d + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ACU_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return e;
+}
+
+// First shift estimate from the centroids of the 0 and 180 degree
+// projections (:231-248), FP32 weights, FP64 sums.
+float centroid_shift(int n, int add_nx, const float* p0, const float* p180) {
+#pragma clang fp contract(off)
+  const int nw = n + add_nx;
+  double tm = 0.0, c0 = 0.0, c180 = 0.0;
+  for (int jx = 1; jx <= n; ++jx) {
+    const float w = (2.0f * (float)nw * (float)(jx - 1)) / (float)(nw - 1) - (float)n - (float)add_nx;
+    for (int jy = 1; jy <= n; ++jy) {
+      const size_t i = (size_t)(jx - 1) + (size_t)(jy - 1) * n;
+      tm += (double)p0[i] + (double)p180[i];
+      c0 += (double)p0[i] * (double)w;
+      c180 += (double)p180[i] * (double)w;
+    }
+  }
+  return -((float)(c0 / tm) + (float)(c180 / tm)) / (2.0f * (float)n);
+}
+
+// The pattern search of :250-364: FP32 parameters and steps as in the
+// reference, each trial's mirror error on the GPU (FP64 sums).
+void find_center(acu_ctx* c, int n, int add_nx, const float* p0h, const float* p180h, float* theta_out,
+                 float* shift_out, int* evals_out) {
+#pragma clang fp contract(off)
+  PrepScratch sc;
+  float *p0 = nullptr, *p180 = nullptr;
+  const size_t bytes = (size_t)n * n * 4;
+  struct Free { float*& a; float*& b; ~Free() { if (a) hipFree(a); if (b) hipFree(b); } } fr{p0, p180};
+  ACU_HIP_CHECK(hipMalloc(&p0, bytes));
+  ACU_HIP_CHECK(hipMalloc(&p180, bytes));
+  ACU_HIP_CHECK(hipMemcpy(p0, p0h, bytes, hipMemcpyHostToDevice));
+  ACU_HIP_CHECK(hipMemcpy(p180, p180h, bytes, hipMemcpyHostToDevice));
+  float best_shift = centroid_shift(n, add_nx, p0h, p180h), best_theta = 0.0f;
+  float dshift = 1e-4f, dtheta = 1e-4f;
+  const int n_sp = 6;
+  const float n_mag = 1.05f, n_min = 2.0f / (float)n_sp;
+  int evals = 1;
+  double best = mirror_error(c, sc, n, add_nx, best_theta, best_shift, p0, p180);
+  while (std::max(dtheta, dshift) > 1e-8f) {
+    double prev = best;
+    for (int jsp = 1; jsp <= n_sp; ++jsp) {
+      const float th = best_theta + dtheta * (float)(jsp - n_sp / 2);
+      const double e = mirror_error(c, sc, n, add_nx, th, best_shift, p0, p180);
+      ++evals;
+      if (e < best) { best_theta = th; best = e; }
+    }
+    dtheta = best < prev ? n_mag * dtheta : n_min * dtheta;
+    prev = best;
+    for (int jsp = 1; jsp <= n_sp; ++jsp) {
+      const float sh = best_shift + dshift * (float)(jsp - n_sp / 2);
+      const double e = mirror_error(c, sc, n, add_nx, best_theta, sh, p0, p180);
+      ++evals;
+      if (e < best) { best_shift = sh; best = e; }
+    }
+    dshift = best < prev ? n_mag * dshift : n_min * dshift;
+  }
+  *theta_out = best_theta;
+  *shift_out = best_shift;
+  if (evals_out) *evals_out = evals;
+}
+
+// Median filter + correction of every projection into the sinogram volume
+// (:384-487): chunks of up to B projections go up with one neighbour on
+// each side (the first and last projections are their own neighbours).
+void prep_sinograms(acu_ctx* c, int n, int nfiles, const float* proj, float theta, float shift, float* sino_h) {
+  const int add_nx = n % 2 == 0 ? 1 : 0, nw = n + add_nx;
+  const size_t nn = (size_t)n * n;
+  const int B = std::max(1, std::min(nfiles, (int)std::max<size_t>(1, ((size_t)1 << 30) / (nn * 4))));
+  float *win = nullptr, *med = nullptr, *sino = nullptr;
+  struct Free {
+    float*& a; float*& b; float*& s;
+    ~Free() { if (a) hipFree(a); if (b) hipFree(b); if (s) hipFree(s); }
+  } fr{win, med, sino};
+  ACU_HIP_CHECK(hipMalloc(&win, (size_t)(B + 2) * nn * 4));
+  ACU_HIP_CHECK(hipMalloc(&med, (size_t)B * nn * 4));
+  ACU_HIP_CHECK(hipMalloc(&sino, (size_t)n * nfiles * nw * 4));
+  const acu::PrepGeom g = acu::prep_geom(n, add_nx, theta, shift);
+  for (int j0 = 0; j0 < nfiles; j0 += B) {
+    const int cnt = std::min(B, nfiles - j0);
+    // window slot w holds projection j0 - 1 + w (clamped to the range)
+    for (int w = 0; w < cnt + 2; ++w) {
+      const int j = std::min(std::max(j0 - 1 + w, 0), nfiles - 1);
+      ACU_HIP_CHECK(hipMemcpyAsync(win + (size_t)w * nn, proj + (size_t)j * nn, nn * 4, hipMemcpyHostToDevice,
+                                   c->stream));
+    }
+    for (int k = 0; k < cnt; ++k)
+      acu::launch_median27(n, win + (size_t)k * nn, win + (size_t)(k + 1) * nn, win + (size_t)(k + 2) * nn,
+                           med + (size_t)k * nn, c->stream);
+    acu::launch_correct(g, med, cnt, j0, nfiles, sino, c->stream);
+    ACU_HIP_CHECK(hipGetLastError());
+  }
+  ACU_HIP_CHECK(hipMemcpyAsync(sino_h, sino, (size_t)n * nfiles * nw * 4, hipMemcpyDeviceToHost, c->stream));
+  ACU_HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+}  // namespace
+
+int acu_prep_median27_dev(acu_ctx* ctx, int n, const float* l, const float* c, const float* r, float* out) {
+  if (!ctx || n < 1 || !l || !c || !r || !out) return fail(ACU_ERR_ARG, "acu_prep_median27_dev: bad argument");
+  ACU_TRY(hipSetDevice(ctx->device); acu::launch_median27(n, l, c, r, out, ctx->stream);
+          ACU_HIP_CHECK(hipGetLastError()));
+}
+
+int acu_prep_correct_dev(acu_ctx* ctx, int n, int add_nx, float theta, float shift, const float* imgs, int count,
+                         int job0, int nfiles, float* sino) {
+  if (!ctx || n < 1 || (add_nx != 0 && add_nx != 1) || !imgs || !sino || count < 1 || job0 < 0 ||
+      job0 + count > nfiles)
+    return fail(ACU_ERR_ARG, "acu_prep_correct_dev: bad argument");
+  ACU_TRY(hipSetDevice(ctx->device);
+          acu::launch_correct(acu::prep_geom(n, add_nx, theta, shift), imgs, count, job0, nfiles, sino, ctx->stream);
+          ACU_HIP_CHECK(hipGetLastError()));
+}
+
+int acu_prep_mirror_error_dev(acu_ctx* ctx, int n, int add_nx, float theta, float shift, const float* p0,
+                              const float* p180, double* err) {
+  if (!ctx || n < 1 || (add_nx != 0 && add_nx != 1) || !p0 || !p180 || !err)
+    return fail(ACU_ERR_ARG, "acu_prep_mirror_error_dev: bad argument");
+  ACU_TRY(hipSetDevice(ctx->device); PrepScratch sc;
+          *err = mirror_error(ctx, sc, n, add_nx, theta, shift, p0, p180));
+}
+
+int acu_prep_find_center(acu_ctx* ctx, int n, const float* p0, const float* p180, float* theta, float* shift,
+                         int* evaluations) {
+  if (!ctx || n < 2 || !p0 || !p180 || !theta || !shift) return fail(ACU_ERR_ARG, "acu_prep_find_center: bad argument");
+  ACU_TRY(hipSetDevice(ctx->device); find_center(ctx, n, n % 2 == 0 ? 1 : 0, p0, p180, theta, shift, evaluations));
+}
+
+int acu_prep_sinograms(acu_ctx* ctx, int n, int nfiles, const float* projections, float theta, float shift,
+                       float* sinograms) {
+  if (!ctx || n < 2 || nfiles < 1 || !projections || !sinograms)
+    return fail(ACU_ERR_ARG, "acu_prep_sinograms: bad argument");
+  ACU_TRY(hipSetDevice(ctx->device); prep_sinograms(ctx, n, nfiles, projections, theta, shift, sinograms));
 }
 
 int acu_tvopt(acu_batch* b, const float* f_host, float mu, float lambda, int nouter, int ninner,

@@ -230,6 +230,39 @@ int acu_volume_reconstruct(acu_ctx *ctx, int nslices, int nimage, int ntheta, in
                            float *tv_host, float mu, float lambda, int nouter, int ninner, int ncg,
                            int batch, int tune, double *stats);
 
+/* ---- Projection -> sinogram preprocessing, the GPU restatement of
+ * tomography/projections_to_sinogram_hpc.F90:231-487.  A projection is
+ * img(x, y), n x n floats (x = detector column fastest, y = detector row =
+ * slice); Nw = n + add_nx with add_nx = 1 for even n (the reference's
+ * add_Nx).  Sinogram volumes are n slices x nfiles angles x Nw, slice-major,
+ * the layout of sinograms{js}.dat. ----
+ *
+ * 3x3x3 median of projection c with its neighbours l and r, neighbours
+ * clamped into the image (:420-447).  Device pointers, asynchronous. */
+int acu_prep_median27_dev(acu_ctx *ctx, int n, const float *l, const float *c, const float *r,
+                          float *out);
+/* Rotation / centre-of-rotation correction by clamped bilinear resampling
+ * (:449-464) of `count` projections (device, n*n each) into rows job0 ..
+ * job0+count-1 of every slice's sinogram in the device volume `sino`
+ * (n x nfiles x Nw).  Asynchronous. */
+int acu_prep_correct_dev(acu_ctx *ctx, int n, int add_nx, float theta, float shift,
+                         const float *imgs, int count, int job0, int nfiles, float *sino);
+/* Mirror error sum |V0(x,y) - V180(Nw-x+1,y)| of the resampled 0 and 180
+ * degree projections at (theta, shift) (:260-283), summed in FP64.
+ * Synchronous. */
+int acu_prep_mirror_error_dev(acu_ctx *ctx, int n, int add_nx, float theta, float shift,
+                              const float *p0_dev, const float *p180_dev, double *err);
+/* Centre-of-rotation and tilt search (:231-364): centroid estimate, then the
+ * reference's pattern search on (theta, shift) with every trial's mirror
+ * error on the GPU.  Host projections in; theta, shift and (may be NULL)
+ * the number of error evaluations out.  Synchronous. */
+int acu_prep_find_center(acu_ctx *ctx, int n, const float *p0_host, const float *p180_host,
+                         float *theta, float *shift, int *evaluations);
+/* Median filter and correction of all nfiles projections (host, nfiles*n*n)
+ * into the sinogram volume (host, n*nfiles*Nw) (:384-487).  Synchronous. */
+int acu_prep_sinograms(acu_ctx *ctx, int n, int nfiles, const float *projections, float theta,
+                       float shift, float *sinograms);
+
 /* Kernel timing: when enabled, every ramp-filter, backprojection, cells and
  * forward-projection launch is bracketed by HIP events on the context's
  * stream (graph mode is then run as ACU_TV_BATCHED).  acu_ctx_kernel_time

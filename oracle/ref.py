@@ -59,6 +59,13 @@ def lib():
             getattr(_lib, f"orc_loss_grad_{p}").argtypes = [
                 ctypes.c_int, ctypes.c_int, _F, ptr, _F, ptr, ctypes.c_double, ctypes.c_double,
                 _D, ptr, ptr, ptr]
+        _lib.orc_prep_median27.argtypes = [ctypes.c_int, _F, _F, _F, _F]
+        _lib.orc_prep_correct.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, _F, _F]
+        _lib.orc_prep_mirror_error.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, _F, _F]
+        _lib.orc_prep_mirror_error.restype = ctypes.c_double
+        _lib.orc_prep_centroid_shift.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F]
+        _lib.orc_prep_centroid_shift.restype = ctypes.c_float
+        _lib.orc_prep_find_center.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F, _F, _F]
         _lib.orc_tvopt_f32d.argtypes = [
             ctypes.c_int, ctypes.c_int, _F, ctypes.c_int, _F, ctypes.c_float, ctypes.c_float,
             ctypes.c_int, ctypes.c_int, ctypes.c_int, _F, _F, _F]
@@ -274,3 +281,52 @@ def loss_grad(x: np.ndarray, theta, shifts, b: np.ndarray, lam: float, eps: floa
     if curvature:
         return loss, gx, gs, hs
     return loss, gx, gs
+
+
+# ---- projection -> sinogram preprocessing (projections_to_sinogram_hpc.F90) ----
+
+def _img(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def prep_median27(l, c, r) -> np.ndarray:
+    """3x3x3 median of projection c with neighbours l, r (:420-447)."""
+    n = c.shape[0]
+    L, C, R = _img(l), _img(c), _img(r)
+    out = np.empty_like(C)
+    lib().orc_prep_median27(n, L.ctypes.data_as(_F), C.ctypes.data_as(_F), R.ctypes.data_as(_F),
+                            out.ctypes.data_as(_F))
+    return out
+
+
+def prep_correct(img, theta: float, shift: float, add_nx=None) -> np.ndarray:
+    """Clamped bilinear rotation / shift correction (:449-464): (n, Nw)."""
+    n = img.shape[0]
+    a = (1 if n % 2 == 0 else 0) if add_nx is None else add_nx
+    x = _img(img)
+    out = np.empty((n, n + a), np.float32)
+    lib().orc_prep_correct(n, a, theta, shift, x.ctypes.data_as(_F), out.ctypes.data_as(_F))
+    return out
+
+
+def prep_mirror_error(p0, p180, theta: float, shift: float, add_nx=None) -> float:
+    n = p0.shape[0]
+    a = (1 if n % 2 == 0 else 0) if add_nx is None else add_nx
+    return lib().orc_prep_mirror_error(n, a, theta, shift, _img(p0).ctypes.data_as(_F),
+                                       _img(p180).ctypes.data_as(_F))
+
+
+def prep_centroid_shift(p0, p180, add_nx=None) -> float:
+    n = p0.shape[0]
+    a = (1 if n % 2 == 0 else 0) if add_nx is None else add_nx
+    return lib().orc_prep_centroid_shift(n, a, _img(p0).ctypes.data_as(_F), _img(p180).ctypes.data_as(_F))
+
+
+def prep_find_center(p0, p180, add_nx=None):
+    """(theta, shift) of the reference's pattern search (:250-364)."""
+    n = p0.shape[0]
+    a = (1 if n % 2 == 0 else 0) if add_nx is None else add_nx
+    th, sh = ctypes.c_float(), ctypes.c_float()
+    lib().orc_prep_find_center(n, a, _img(p0).ctypes.data_as(_F), _img(p180).ctypes.data_as(_F),
+                               ctypes.byref(th), ctypes.byref(sh))
+    return th.value, sh.value
