@@ -1,5 +1,6 @@
 """Builds the Fortran module `Tomography` (iso_c_binding over libacumen_hip.so)
-and the drivers fbp_hip, tv_hip, tv_hip_slice into acumen_amd/fortran/bin/.
+and the drivers fbp_hip, tv_hip, tv_hip_slice, projections_to_sinogram_hip into
+acumen_amd/fortran/bin/.
 
 Usage: python -m acumen_amd.fortran_build
 """
@@ -15,7 +16,7 @@ FDIR = os.path.join(HERE, "fortran")
 BUILD = os.path.join(FDIR, "_build")
 BIN = os.path.join(FDIR, "bin")
 MODULES = ["acumen_io.F90", "acumen_tomography.F90"]
-DRIVERS = ["fbp_hip.F90", "tv_hip.F90", "tv_hip_slice.F90"]
+DRIVERS = ["fbp_hip.F90", "tv_hip.F90", "tv_hip_slice.F90", "projections_to_sinogram_hip.F90"]
 FFLAGS = ["-O2"]
 
 

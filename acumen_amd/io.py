@@ -91,6 +91,48 @@ def write_dataset(ds: Dataset, theta_deg, sinograms, n_files_sino: int = 1,
         write_input_parm(ds, **parm)
 
 
+def write_projections(ds: Dataset, theta_deg, projections, n_files_sino: int = 1,
+                      n_files_rec: int = 1) -> None:
+    """The input of projections_to_sinogram: projections (nfiles, nimage,
+    nimage) img(x, y) with x fastest, in projections{js}.dat split as the
+    reference reads them (j = ceiling(job / r), offset = (job - floor((j-1) r)
+    - 1) nimage^2, r = nfiles / n_files_sino;
+    projections_to_sinogram_hpc.F90:391-407), theta.dat and the 4-float
+    header [nfiles, nimage, n_files_sino, n_files_rec] (:56-71)."""
+    os.makedirs(ds.rec, exist_ok=True)
+    theta = _f32(theta_deg)
+    proj = _f32(projections)
+    nfiles, n, n2 = proj.shape
+    assert n == n2 and nfiles == theta.size
+    np.array([nfiles, n, n_files_sino, n_files_rec], np.float32).tofile(ds.path("fileinfo_projections.dat"))
+    theta.tofile(ds.path("theta.dat"))
+    r = np.float32(nfiles) / np.float32(n_files_sino)
+    files = {}
+    for job in range(1, nfiles + 1):
+        j = int(np.ceil(np.float32(job) / r))
+        off = (job - int(np.floor(np.float32(j - 1) * r)) - 1) * n * n
+        files.setdefault(j, []).append((off, proj[job - 1]))
+    for j, parts in files.items():
+        size = max(o for o, _ in parts) + n * n
+        buf = np.zeros(size, np.float32)
+        for o, a in parts:
+            buf[o:o + n * n] = a.ravel()
+        buf.tofile(ds.path(f"projections{j}.dat"))
+
+
+def read_sinograms(ds: Dataset, nimage: int, nfiles: int, n_files_sino: int = 1) -> np.ndarray:
+    """All slices of sinograms{js}.dat as (nimage, nfiles, Nw)."""
+    nw = nimage + (1 if nimage % 2 == 0 else 0)
+    out = np.zeros((nimage, nfiles, nw), np.float32)
+    cache = {}
+    for s in range(1, nimage + 1):
+        js, off = slice_location(s, nimage, n_files_sino, nw * nfiles)
+        if js not in cache:
+            cache[js] = np.fromfile(ds.path(f"sinograms{js}.dat"), np.float32)
+        out[s - 1] = cache[js][off:off + nw * nfiles].reshape(nfiles, nw)
+    return out
+
+
 def write_input_parm(ds: Dataset, nsub=1, nsub_t=1, ninner=1, nouter=1, ncg=5,
                      lam=0.1, mu=0.05, wpu=4) -> None:
     with open(os.path.join(ds.root, ds.name, "input_parm.txt"), "w") as fh:

@@ -103,3 +103,24 @@ def test_tv_volume_driver(ctx, volume, nsub, nsub_t):
             b.tvopt(MU, LAM, NOUTER, 1, NCG, ACU_TV_GRAPH)
             assert np.array_equal(tvs[s], b.get_image(0, ACU_IMG_TV))
             assert np.array_equal(fbps[s], b.get_image(0, ACU_IMG_FBP))
+
+
+def test_projections_to_sinogram_driver(ctx, tmp_path):
+    # angles without an exact 0 / 180 (both end projections interpolated),
+    # two projection and sinogram files, even n (Nw = n + 1)
+    from acumen_amd import prep
+    from test_oracle_prep import blob_image
+    n, nfiles, nfs = 32, 25, 2
+    th = np.linspace(-1.3, 181.2, nfiles).astype(np.float32)
+    rng = np.random.default_rng(4)
+    proj = np.stack([blob_image(n, 50 + j) + rng.normal(0, 0.01, (n, n)).astype(np.float32)
+                     for j in range(nfiles)])
+    ds = aio.Dataset(str(tmp_path), "raw")
+    aio.write_projections(ds, th, proj, n_files_sino=nfs)
+    out = run("projections_to_sinogram_hip", ds)
+    assert "Best theta:" in out
+    got = aio.read_sinograms(ds, n, nfiles, n_files_sino=nfs)
+    p0 = prep.angle_window_projection(proj, th, 0.0)
+    p180 = prep.angle_window_projection(proj, th, 180.0)
+    t, s, _ = prep.find_center(ctx, p0, p180)
+    assert np.array_equal(got, prep.sinograms(ctx, proj, t, s))
