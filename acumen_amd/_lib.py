@@ -24,7 +24,7 @@ ACU_TV_PLAIN = 0
 ACU_TV_BATCHED = 1
 ACU_TV_GRAPH = 2
 
-KIND_RAMP, KIND_BP, KIND_CELLS, KIND_FP, KIND_ADJ = 0, 1, 2, 3, 4
+KIND_RAMP, KIND_BP, KIND_CELLS, KIND_FP, KIND_ADJ, KIND_PREP, KIND_INGEST = 0, 1, 2, 3, 4, 5, 6
 RAMP_FP64, RAMP_FP32, RAMP_BF16X3, RAMP_BF16X6, RAMP_FFT = 0, 1, 2, 3, 4
 
 # Every symbol the header declares, with (restype, argtypes).
@@ -79,6 +79,10 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_double)]),
     "acu_prep_find_center": (_c_int, [_vp, _c_int, _fp, _fp, _fp, _fp, ctypes.POINTER(_c_int)]),
     "acu_prep_sinograms": (_c_int, [_vp, _c_int, _c_int, _fp, _c_float, _c_float, _fp]),
+    "acu_ingest_mean_dev": (_c_int, [_vp, _c_int, ctypes.c_longlong, _vp, _vp]),
+    "acu_ingest_normalize_dev": (_c_int, [_vp, _c_int, ctypes.c_longlong, _vp, _vp, _vp]),
+    "acu_ingest_mean": (_c_int, [_vp, _c_int, ctypes.c_longlong, _vp, _vp]),
+    "acu_ingest_normalize": (_c_int, [_vp, _c_int, ctypes.c_longlong, _vp, _vp, _vp]),
     "acu_ctx_set_timing": (_c_int, [_vp, _c_int]),
     "acu_ctx_kernel_time": (_c_int, [_vp, _c_int, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_longlong)]),

@@ -21,5 +21,8 @@ void launch_correct(const PrepGeom& g, const float* imgs, int count, int job0, i
 int mirror_blocks(const PrepGeom& g);
 void launch_mirror(const PrepGeom& g, const float* p0, const float* p180, double* part, double* out,
                    hipStream_t s);
+// FITS ingest normalisation (fits_to_hpc.F90:196-248)
+void launch_mean(const float* imgs, int count, size_t npix, float* out, hipStream_t s);
+void launch_normalize(const float* imgs, int count, size_t npix, const float* ob, float* out, hipStream_t s);
 
 }  // namespace acu

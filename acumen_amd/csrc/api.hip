@@ -36,9 +36,10 @@ struct acu_ctx {
   struct Pending { int kind; hipEvent_t a, b; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
-  static constexpr int KINDS = 5;  // ramp, backprojection, cells, forward, adjoint
-  double ms[KINDS] = {0, 0, 0, 0, 0};
-  long long count[KINDS] = {0, 0, 0, 0, 0};
+  // ramp, backprojection, cells, forward, adjoint, preprocessing, ingest
+  static constexpr int KINDS = 7;
+  double ms[KINDS] = {0, 0, 0, 0, 0, 0, 0};
+  long long count[KINDS] = {0, 0, 0, 0, 0, 0, 0};
 
   hipEvent_t ev() {
     if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
@@ -256,7 +257,7 @@ struct acu_batch {
 namespace acu {
 namespace {
 
-enum Kind { K_RF = 0, K_BP = 1, K_CELLS = 2, K_FP = 3, K_ADJ = 4 };
+enum Kind { K_RF = 0, K_BP = 1, K_CELLS = 2, K_FP = 3, K_ADJ = 4, K_PREP = 5, K_INGEST = 6 };
 
 template <class F>
 void timed(acu_ctx* c, int kind, F&& launch) {
@@ -1658,14 +1659,16 @@ void prep_sinograms(acu_ctx* c, int n, int nfiles, const float* proj, float thet
       ACU_HIP_CHECK(hipMemcpyAsync(win + (size_t)w * nn, proj + (size_t)j * nn, nn * 4, hipMemcpyHostToDevice,
                                    c->stream));
     }
-    for (int k = 0; k < cnt; ++k)
-      acu::launch_median27(n, win + (size_t)k * nn, win + (size_t)(k + 1) * nn, win + (size_t)(k + 2) * nn,
-                           med + (size_t)k * nn, c->stream);
-    acu::launch_correct(g, med, cnt, j0, nfiles, sino, c->stream);
+    acu::timed(c, acu::K_PREP, [&] {
+      for (int k = 0; k < cnt; ++k)
+        acu::launch_median27(n, win + (size_t)k * nn, win + (size_t)(k + 1) * nn, win + (size_t)(k + 2) * nn,
+                             med + (size_t)k * nn, c->stream);
+      acu::launch_correct(g, med, cnt, j0, nfiles, sino, c->stream);
+    });
     ACU_HIP_CHECK(hipGetLastError());
   }
   ACU_HIP_CHECK(hipMemcpyAsync(sino_h, sino, (size_t)n * nfiles * nw * 4, hipMemcpyDeviceToHost, c->stream));
-  ACU_HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->sync();
 }
 }  // namespace
 
@@ -1704,6 +1707,46 @@ int acu_prep_sinograms(acu_ctx* ctx, int n, int nfiles, const float* projections
   if (!ctx || n < 2 || nfiles < 1 || !projections || !sinograms)
     return fail(ACU_ERR_ARG, "acu_prep_sinograms: bad argument");
   ACU_TRY(hipSetDevice(ctx->device); prep_sinograms(ctx, n, nfiles, projections, theta, shift, sinograms));
+}
+
+int acu_ingest_mean_dev(acu_ctx* ctx, int count, long long npix, const float* imgs, float* out) {
+  if (!ctx || count < 1 || npix < 1 || !imgs || !out) return fail(ACU_ERR_ARG, "acu_ingest_mean_dev: bad argument");
+  ACU_TRY(hipSetDevice(ctx->device); acu::timed(ctx, acu::K_INGEST, [&] { acu::launch_mean(imgs, count, (size_t)npix, out, ctx->stream); });
+          ACU_HIP_CHECK(hipGetLastError()));
+}
+
+int acu_ingest_normalize_dev(acu_ctx* ctx, int count, long long npix, const float* imgs, const float* open_beam,
+                             float* out) {
+  if (!ctx || count < 1 || npix < 1 || !imgs || !open_beam || !out)
+    return fail(ACU_ERR_ARG, "acu_ingest_normalize_dev: bad argument");
+  ACU_TRY(hipSetDevice(ctx->device); acu::timed(ctx, acu::K_INGEST,
+                     [&] { acu::launch_normalize(imgs, count, (size_t)npix, open_beam, out, ctx->stream); });
+          ACU_HIP_CHECK(hipGetLastError()));
+}
+
+int acu_ingest_mean(acu_ctx* ctx, int count, long long npix, const float* imgs, float* out) {
+  if (!ctx || count < 1 || npix < 1 || !imgs || !out) return fail(ACU_ERR_ARG, "acu_ingest_mean: bad argument");
+  ACU_TRY(hipSetDevice(ctx->device);
+          const size_t np = (size_t)npix;
+          DevBuf in(np * count), o(np);
+          ACU_HIP_CHECK(hipMemcpyAsync(in.p, imgs, np * count * 4, hipMemcpyHostToDevice, ctx->stream));
+          acu::timed(ctx, acu::K_INGEST, [&] { acu::launch_mean(in.p, count, np, o.p, ctx->stream); }); ACU_HIP_CHECK(hipGetLastError());
+          ACU_HIP_CHECK(hipMemcpyAsync(out, o.p, np * 4, hipMemcpyDeviceToHost, ctx->stream));
+          ctx->sync());
+}
+
+int acu_ingest_normalize(acu_ctx* ctx, int count, long long npix, const float* imgs, const float* open_beam,
+                         float* out) {
+  if (!ctx || count < 1 || npix < 1 || !imgs || !open_beam || !out)
+    return fail(ACU_ERR_ARG, "acu_ingest_normalize: bad argument");
+  ACU_TRY(hipSetDevice(ctx->device);
+          const size_t np = (size_t)npix;
+          DevBuf in(np * count), ob(np), o(np * count);
+          ACU_HIP_CHECK(hipMemcpyAsync(in.p, imgs, np * count * 4, hipMemcpyHostToDevice, ctx->stream));
+          ACU_HIP_CHECK(hipMemcpyAsync(ob.p, open_beam, np * 4, hipMemcpyHostToDevice, ctx->stream));
+          acu::timed(ctx, acu::K_INGEST, [&] { acu::launch_normalize(in.p, count, np, ob.p, o.p, ctx->stream); }); ACU_HIP_CHECK(hipGetLastError());
+          ACU_HIP_CHECK(hipMemcpyAsync(out, o.p, np * count * 4, hipMemcpyDeviceToHost, ctx->stream));
+          ctx->sync());
 }
 
 int acu_tvopt(acu_batch* b, const float* f_host, float mu, float lambda, int nouter, int ninner,

@@ -10,6 +10,8 @@
 // slice jy, Nw = n + add_Nx values long.
 #include "acu_prep.hpp"
 
+#include <algorithm>
+
 namespace acu {
 namespace {
 
@@ -174,6 +176,129 @@ PrepGeom prep_geom(int n, int add_nx, float theta, float shift) {
   g.c = cosf(theta);
   g.s = sinf(theta);
   return g;
+}
+
+}  // namespace acu
+
+// ---- FITS ingest normalisation (tomography/fits_to_hpc.F90:196-248) ----
+//
+// mean: one thread per pixel (VEC: four adjacent pixels, one 16-byte
+// access) loops over the images in file order.
+//
+// normalize: -log(I / b), b = max(OB, 1).  The open beam is fixed per pixel
+// across the projections, so its reciprocal y = RN(1/b) is formed once per
+// pixel with the IEEE division and each quotient then costs one product and
+// two FMAs:  q = RN(I y),  q' = RN(q + RN(I - b q) y).  With y correctly
+// rounded and q within one ulp of I/b, the remainder I - b q is exact and q'
+// is the correctly rounded quotient (P. Markstein, IBM J. Res. Dev. 34(1),
+// 1990; Cornea-Hasegan, Golliver, Markstein, ARITH-14, 1999, Theorem 1),
+// i.e. bit-identical to I / b, provided nothing underflows or overflows: for
+// |I| < 2^-100, |q| < 2^-100, b = inf, or I not finite the kernel takes the
+// IEEE division instead.  Each thread handles one pixel group of
+// ING_CHUNK consecutive projections (grid: pixel groups x projection
+// chunks); longer per-thread runs across images measured slower (the
+// write streams lose DRAM locality).  VEC needs npix % 4 == 0 and 16-byte
+// aligned pointers (checked on the host).
+namespace acu {
+namespace {
+
+constexpr int ING_CHUNK = 2;
+
+template <bool VEC> struct Px { typedef float type; };
+template <> struct Px<true> { typedef float4 type; };
+
+__device__ __forceinline__ float add_(float a, float b) { return a + b; }
+__device__ __forceinline__ float4 add_(float4 a, float4 b) {
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ float div_(float a, float c) { return a / c; }
+__device__ __forceinline__ float4 div_(float4 a, float c) { return make_float4(a.x / c, a.y / c, a.z / c, a.w / c); }
+
+struct BY { float b, y; };
+__device__ __forceinline__ BY by_(float ob) {
+  const float b = fmaxf(ob, 1.0f);
+  return {b, 1.0f / b};
+}
+__device__ __forceinline__ float nlog_(float a, BY p) {
+  float q = a * p.y;
+  if (fabsf(a) >= 0x1p-100f && fabsf(a) <= 3.40282347e38f && fabsf(q) >= 0x1p-100f && p.b <= 3.40282347e38f)
+    q = fmaf(fmaf(-p.b, q, a), p.y, q);
+  else
+    q = a / p.b;
+  return -logf(q);
+}
+
+// out = (sum over images, in order, FP32) / count: the open-beam average of
+// :203-215, accumulated one file at a time from bg = 0
+template <bool VEC>
+__global__ __launch_bounds__(256) void mean_kernel(const float* __restrict__ imgs, int count, size_t npix,
+                                                   float* __restrict__ out) {
+#pragma clang fp contract(off)
+  typedef typename Px<VEC>::type P;
+  const size_t n = VEC ? npix / 4 : npix;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const P* src = reinterpret_cast<const P*>(imgs);
+  P s;
+  if constexpr (VEC) s = make_float4(0.0f, 0.0f, 0.0f, 0.0f); else s = 0.0f;  // -0 + ... gives +0, as bg = 0
+  for (int k = 0; k < count; ++k) s = add_(s, src[(size_t)k * n + i]);
+  reinterpret_cast<P*>(out)[i] = div_(s, (float)count);
+}
+
+// projection = -log(image / max(open_beam, 1)) (:243); blockIdx.y = chunk
+template <bool VEC>
+__global__ __launch_bounds__(256) void normalize_kernel(const float* __restrict__ imgs, int count, size_t npix,
+                                                        const float* __restrict__ ob, float* __restrict__ out) {
+#pragma clang fp contract(off)
+  const size_t n = VEC ? npix / 4 : npix;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k0 = blockIdx.y * ING_CHUNK, k1 = min(count, k0 + ING_CHUNK);
+  if constexpr (VEC) {
+    const float4 o = reinterpret_cast<const float4*>(ob)[i];
+    const BY p0 = by_(o.x), p1 = by_(o.y), p2 = by_(o.z), p3 = by_(o.w);
+    const float4* src = reinterpret_cast<const float4*>(imgs);
+    float4* dst = reinterpret_cast<float4*>(out);
+    for (int k = k0; k < k1; ++k) {
+      const float4 v = src[(size_t)k * n + i];
+      dst[(size_t)k * n + i] = make_float4(nlog_(v.x, p0), nlog_(v.y, p1), nlog_(v.z, p2), nlog_(v.w, p3));
+    }
+  } else {
+    const BY p = by_(ob[i]);
+    for (int k = k0; k < k1; ++k) out[(size_t)k * n + i] = nlog_(imgs[(size_t)k * n + i], p);
+  }
+}
+
+bool vec_ok(size_t npix, const void* a, const void* b, const void* c) {
+  const uintptr_t m = (uintptr_t)a | (uintptr_t)b | (uintptr_t)c;
+  return npix % 4 == 0 && (m & 15) == 0;
+}
+
+}  // namespace
+
+void launch_mean(const float* imgs, int count, size_t npix, float* out, hipStream_t s) {
+  if (vec_ok(npix, imgs, out, out))
+    hipLaunchKernelGGL(mean_kernel<true>, dim3((unsigned)((npix / 4 + 255) / 256)), dim3(256), 0, s, imgs, count,
+                       npix, out);
+  else
+    hipLaunchKernelGGL(mean_kernel<false>, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, imgs, count,
+                       npix, out);
+}
+
+void launch_normalize(const float* imgs, int count, size_t npix, const float* ob, float* out, hipStream_t s) {
+  const bool vec = vec_ok(npix, imgs, ob, out);
+  const unsigned gx = (unsigned)(((vec ? npix / 4 : npix) + 255) / 256);
+  constexpr int kMaxImages = 65535 * ING_CHUNK;  // gridDim.y limit
+  for (int k = 0; k < count; k += kMaxImages) {
+    const int c = std::min(kMaxImages, count - k);
+    const dim3 grid(gx, (unsigned)((c + ING_CHUNK - 1) / ING_CHUNK));
+    const float* in = imgs + (size_t)k * npix;
+    float* o = out + (size_t)k * npix;
+    if (vec)
+      hipLaunchKernelGGL(normalize_kernel<true>, grid, dim3(256), 0, s, in, c, npix, ob, o);
+    else
+      hipLaunchKernelGGL(normalize_kernel<false>, grid, dim3(256), 0, s, in, c, npix, ob, o);
+  }
 }
 
 }  // namespace acu

@@ -67,14 +67,16 @@ def fp_kernel_name():
     return "fp_kernel" if os.environ.get("ACUMEN_FP_PATH") == "gather" else "fpl_kernel"
 
 
-def traffic_from_profile():
-    """HBM bytes per forward-projection launch from the committed rocprofv3
-    counter summary, if one is present."""
-    path = os.path.join(ROOT, "profiles", f"{fp_kernel_name()}_traffic_2049x1500.txt")
+def traffic_from_profile(name=None, key="hbm_bytes_per_launch_per_slice"):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 counter
+    summary under profiles/, if one is present (default: the forward
+    projector at 2049 x 1500, per slice)."""
+    name = name or f"{fp_kernel_name()}_traffic_2049x1500.txt"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("hbm_bytes_per_launch_per_slice:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -332,10 +334,108 @@ def run_prep(args) -> int:
     return 0
 
 
+def run_ingest(args) -> int:
+    """FITS ingest normalisation, -log(I / max(OB, 1)), of 256 projections
+    of 2048^2 resident in HBM (the GPU part of tomography/fits_to_hpc.F90).
+    A step is one launch over the whole batch.  Algorithmic traffic: each
+    projection read and written once (8 B per pixel) plus the open beam
+    read once per launch; the launch duration comes from HIP events on the
+    library's stream.  The host-buffer form (PCIe included) and the oracle
+    on the host cores are reported beside it."""
+    import numpy as np
+    import torch
+
+    from acumen_amd import Context, _lib, ingest
+    from oracle import ref
+
+    n, count = N_DET, 256
+    npix = n * n
+    ctx = Context(0)
+    lib = _lib.load()
+    dev = torch.device("cuda", 0)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4000)
+    ob = (torch.rand((n, n), generator=gen, device=dev) * 4000.0 + 18000.0).floor()
+    trans = torch.rand((count, n, n), generator=gen, device=dev) * 0.9 + 0.05
+    imgs = (ob * trans).floor()
+    del trans
+    out = torch.empty_like(imgs)
+    torch.cuda.synchronize()
+
+    def launch():
+        _lib.check(lib.acu_ingest_normalize_dev(ctx.handle, count, npix, imgs.data_ptr(), ob.data_ptr(),
+                                                out.data_ptr()))
+
+    log("ingest normalisation:", count, "projections of", n, "^2 in HBM")
+    for _ in range(args.warmup):
+        launch()
+    ctx.synchronize()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        launch()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    k_ms, k_n = ctx.kernel_time(_lib.KIND_INGEST)
+    ctx.set_timing(False)
+    avg_s = k_ms / max(k_n, 1) / 1e3
+    alg_bytes = 8.0 * npix * count + 4.0 * npix
+    # parity sample against the oracle (2 projections)
+    got = out[:2].cpu().numpy()
+    want = ref.ingest_normalize(imgs[:2].cpu().numpy(), ob.cpu().numpy())
+    gi = got.view(np.int32).astype(np.int64)
+    wi = want.view(np.int32).astype(np.int64)
+    ulps = int(np.abs(gi - wi).max())
+    # host-buffer form (PCIe both ways), 32 projections
+    h_imgs, h_ob = imgs[:32].cpu().numpy(), ob.cpu().numpy()
+    norm = ingest.Normalizer(ctx)
+    norm.normalize(h_imgs[:2], h_ob)
+    t = time.perf_counter()
+    norm.normalize(h_imgs, h_ob)
+    pcie = 32 / (time.perf_counter() - t)
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline: oracle normalisation, one thread")
+        done, t = 0, time.perf_counter()
+        while done < 256 and time.perf_counter() - t < 10.0:
+            ref.ingest_normalize(h_imgs[done % 32:done % 32 + 16], h_ob)
+            done += 16
+        el = time.perf_counter() - t
+        cpu = {"value": round(done / el, 2), "unit": "projections/s", "cores": 1, "kind": "port",
+               "sample": f"{done} projections of {n}^2 through the oracle's FP32 loop (C, logf), "
+                         f"one thread, {el:.1f} s"}
+    ctx.close()
+    traffic = traffic_from_profile("ingest_normalize_traffic_2048x256.txt", "hbm_bytes_per_launch")
+    res = {
+        "metric": "projections/s through -log(I / max(OB, 1)), 2048^2, inputs in HBM",
+        "value": round(count * args.steps / wall, 1),
+        "unit": "projections/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / max(args.steps, 1) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"ingest normalisation of {count} projections of {n}^2 (open beam ~2e4 counts)",
+                   "n": n, "projections": count},
+        "roofline": {"bound": "hbm", "achieved": round(alg_bytes / avg_s / 1e9, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(alg_bytes / avg_s / 1e9 / 8000.0, 4), "traffic": traffic},
+        "kernel_ms": round(avg_s * 1e3, 4),
+        "pcie_incl_projections_per_s": round(pcie, 1),
+        "max_ulps_vs_oracle_sample": ulps,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 2; ingest: 50)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=4, help="slices per step")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-slice leg")
@@ -344,14 +444,17 @@ def main() -> int:
     ap.add_argument("--nouter", type=int, default=NOUTER,
                     help="outer iterations (default: the workload's 50; fewer only for profiling runs)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
-    ap.add_argument("--workload", choices=("production", "mixed", "volume", "prep"), default="production",
+    ap.add_argument("--workload", choices=("production", "mixed", "volume", "prep", "ingest"), default="production",
                     help="production: 2048^2 x 1500 slices (the headline); mixed: 64 slices of "
                          "uneven width and angle count in one batch; volume: a 2048^2 x 1500 x 256 "
-                         "volume streamed through FBP (+ TV on --tv-slices slices)")
+                         "volume streamed through FBP (+ TV on --tv-slices slices); prep: projection -> "
+                         "sinogram preprocessing; ingest: FITS ingest normalisation")
     ap.add_argument("--slices", type=int, default=256, help="volume workload: slices streamed through FBP")
     ap.add_argument("--tv-slices", type=int, default=16,
                     help="volume workload: slices streamed through FBP + 30-iteration TV")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 50 if args.workload == "ingest" else 2
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(slices are independent; multi-GPU sharding is described in DESIGN.md)",
@@ -367,6 +470,8 @@ def main() -> int:
         return run_volume(args)
     if args.workload == "prep":
         return run_prep(args)
+    if args.workload == "ingest":
+        return run_ingest(args)
 
     import numpy as np
 

@@ -263,12 +263,32 @@ int acu_prep_find_center(acu_ctx *ctx, int n, const float *p0_host, const float 
 int acu_prep_sinograms(acu_ctx *ctx, int n, int nfiles, const float *projections, float theta,
                        float shift, float *sinograms);
 
-/* Kernel timing: when enabled, every ramp-filter, backprojection, cells and
- * forward-projection launch is bracketed by HIP events on the context's
+/* ---- FITS ingest normalisation, the GPU part of tomography/fits_to_hpc.F90
+ * (the FITS decoding itself is host I/O). ----
+ *
+ * Mean of `count` images of npix pixels (device, image after image),
+ * accumulated in file order in FP32 and divided by count, as the open-beam
+ * average of :203-213.  Asynchronous. */
+int acu_ingest_mean_dev(acu_ctx *ctx, int count, long long npix, const float *imgs, float *out);
+/* out = -log(image / max(open_beam, 1)) for `count` images (:243).
+ * Asynchronous. */
+int acu_ingest_normalize_dev(acu_ctx *ctx, int count, long long npix, const float *imgs,
+                             const float *open_beam, float *out);
+
+/* Host-buffer forms of the two above (staged through device memory;
+ * synchronous).  These are what the ingest driver calls per batch of
+ * decoded FITS images. */
+int acu_ingest_mean(acu_ctx *ctx, int count, long long npix, const float *imgs, float *out);
+int acu_ingest_normalize(acu_ctx *ctx, int count, long long npix, const float *imgs,
+                         const float *open_beam, float *out);
+
+/* Kernel timing: when enabled, every ramp-filter, backprojection, cells,
+ * forward-projection, preprocessing and ingest launch is bracketed by HIP events on the context's
  * stream (graph mode is then run as ACU_TV_BATCHED).  acu_ctx_kernel_time
  * reports the accumulated milliseconds and launch count of one kind:
  * 0 = ramp filter, 1 = backprojection, 2 = cells pass, 3 = forward projection,
- * 4 = adjoint projection. */
+ * 4 = adjoint projection, 5 = preprocessing (median filter + correction of
+ * one batch of projections), 6 = ingest (open-beam mean, normalisation). */
 int acu_ctx_set_timing(acu_ctx *ctx, int enable);
 int acu_ctx_kernel_time(acu_ctx *ctx, int kind, double *total_ms, long long *launches);
 

@@ -140,3 +140,27 @@ void orc_prep_find_center(int n, int add_nx, const float *p0, const float *p180,
     *theta_out = best_theta;
     *shift_out = best_shift;
 }
+
+/* ---- FITS ingest normalisation (tomography/fits_to_hpc.F90) ---- */
+
+/* Open-beam average of :203-215: bg = 0; bg = bg + image for each file in
+ * order; bg = bg / nfiles_bg, all FP32. */
+void orc_ingest_mean(int count, long long npix, const float *imgs, float *out)
+{
+    for (long long i = 0; i < npix; ++i) {
+        float s = 0.0f;
+        for (int k = 0; k < count; ++k) s = s + imgs[(long long)k * npix + i];
+        out[i] = s / (float)count;
+    }
+}
+
+/* projection = -log(image / max(bg, 1.0)) of :243; gfortran evaluates the
+ * FP32 log with the C library's logf, as here. */
+void orc_ingest_normalize(int count, long long npix, const float *imgs, const float *bg, float *out)
+{
+    for (int k = 0; k < count; ++k)
+        for (long long i = 0; i < npix; ++i) {
+            const float b = bg[i] > 1.0f ? bg[i] : 1.0f;
+            out[(long long)k * npix + i] = -logf(imgs[(long long)k * npix + i] / b);
+        }
+}

@@ -65,6 +65,8 @@ def lib():
         _lib.orc_prep_mirror_error.restype = ctypes.c_double
         _lib.orc_prep_centroid_shift.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F]
         _lib.orc_prep_centroid_shift.restype = ctypes.c_float
+        _lib.orc_ingest_mean.argtypes = [ctypes.c_int, ctypes.c_longlong, _F, _F]
+        _lib.orc_ingest_normalize.argtypes = [ctypes.c_int, ctypes.c_longlong, _F, _F, _F]
         _lib.orc_prep_find_center.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F, _F, _F]
         _lib.orc_tvopt_f32d.argtypes = [
             ctypes.c_int, ctypes.c_int, _F, ctypes.c_int, _F, ctypes.c_float, ctypes.c_float,
@@ -330,3 +332,26 @@ def prep_find_center(p0, p180, add_nx=None):
     lib().orc_prep_find_center(n, a, _img(p0).ctypes.data_as(_F), _img(p180).ctypes.data_as(_F),
                                ctypes.byref(th), ctypes.byref(sh))
     return th.value, sh.value
+
+
+# ---- FITS ingest normalisation (fits_to_hpc.F90) ----
+
+
+def ingest_mean(imgs) -> np.ndarray:
+    """Open-beam average of fits_to_hpc.F90:203-215 over imgs (count, ...)."""
+    a = np.ascontiguousarray(imgs, np.float32)
+    out = np.empty(a.shape[1:], np.float32)
+    lib().orc_ingest_mean(a.shape[0], out.size, a.ctypes.data_as(_F), out.ctypes.data_as(_F))
+    return out
+
+
+def ingest_normalize(imgs, bg) -> np.ndarray:
+    """-log(image / max(bg, 1)) of fits_to_hpc.F90:243 for imgs (count, ...)."""
+    a = np.ascontiguousarray(imgs, np.float32)
+    b = np.ascontiguousarray(bg, np.float32)
+    if b.shape != a.shape[1:]:
+        raise ValueError("bg must have the shape of one image")
+    out = np.empty_like(a)
+    lib().orc_ingest_normalize(a.shape[0], b.size, a.ctypes.data_as(_F), b.ctypes.data_as(_F),
+                               out.ctypes.data_as(_F))
+    return out
