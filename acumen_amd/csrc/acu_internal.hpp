@@ -86,6 +86,15 @@ static_assert(FPL_PITCH + 1 <= 255, "ds_read2 offsets");
 constexpr int FPL_NPITCH = 8;
 constexpr int FPL_PITCHES[FPL_NPITCH] = {93, 97, 99, 101, 103, 105, 107, 109};
 constexpr int FPL_PCLASS_DEFAULT = 3;
+// Cell-layout LDS projector (fplc.hip): the box is FPLC_PITCH x FPLC_ROWS
+// 16-byte bilinear cells (161,600 bytes of the CU's 160 KiB LDS).
+constexpr int FPLC_PITCH = 101;
+constexpr int FPLC_ROWS = 100;
+constexpr int FPLC_CAP = FPLC_PITCH * FPLC_ROWS;
+// Projector variants (per slice): 0 = global gathers (fp.hip), 1/2/4 = LDS
+// boxes with that many angles per lane (fpl.hip), FPV_CELL = cell boxes,
+// one angle per wave, 16 waves (fplc.hip).
+constexpr int FPV_CELL = 8;
 static_assert(FPL_PITCHES[FPL_PCLASS_DEFAULT] == FPL_PITCH, "default pitch class");
 
 // ---- ramp filter GEMM (filter.hip) ----
@@ -130,8 +139,8 @@ struct FPLTask {
   int N, T;
   float mid;
   int block_begin;       // first global block index of this slice
-  int al;                // angles per lane (1, 2 or 4, the same for every slice of a launch)
-  int nab;               // angle blocks = ceil(T / (4*al)); block = rb * nab + ab
+  int al;                // angles per lane (1, 2 or 4), or FPV_CELL; the same for every slice of a launch
+  int nab;               // angle blocks = ceil(T / fpl_angle_block(al)); block = rb * nab + ab
   const float* img;      // N*N image (raw pixels, staged in LDS)
   const float4* cells;   // x-major cells of the same image (fallback for oversized boxes)
   const float2* sc;      // per-angle (sin, cos) for projection
@@ -183,8 +192,10 @@ void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, 
 void launch_cells(const CellTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
 void launch_forward(const FPTask* d_tasks, int ntasks, int base, int blocks, int sum_base,
                     int sum_blocks, hipStream_t s);
+// al = 1, 2, 4: fpl.hip; al = FPV_CELL: fplc.hip
 void launch_forward_lds(const FPLTask* d_tasks, int ntasks, int base, int blocks, int al,
                         hipStream_t s);
+void launch_forward_cell(const FPLTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
 
 // Host-side helpers.
 int bp_tiles_x(int N);
@@ -196,9 +207,13 @@ int fp_chunks(int N, int clen);
 int fp_sum_blocks(int N, int T);
 int fpl_angles_per_lane(int N, int T);
 int fpl_blocks(int N, int T, int al);
+int fplc_blocks(int N, int T);
+// angles per workgroup block of an LDS projector variant
+inline int fpl_angle_block(int al) { return al == FPV_CELL ? 16 : 4 * al; }
 // true when some chunk of the LDS projector can outgrow its box and read
 // the global bilinear cells (which then have to be built first)
 bool fpl_may_overflow(int N, int T, const float* theta_deg, int al);
+bool fplc_may_overflow(int N, int T, const float* theta_deg);
 // Pitch class of every block of 4*al angles: the pitch with the fewest LDS
 // bank conflicts for the block's angle among those whose rows hold the
 // block's box (out: ceil(T / (4*al)) bytes)

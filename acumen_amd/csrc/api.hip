@@ -196,7 +196,7 @@ struct RFSet {
 // slice appears in the cells array only when its variant reads the cells.
 struct FPSet {
   DevArray<CellTask> cl;
-  DevArray<FPLTask> fl[3];
+  DevArray<FPLTask> fl[4];
   DevArray<FPTask> fp;
   void release() {
     cl.release();
@@ -205,7 +205,9 @@ struct FPSet {
   }
 };
 
-inline int fl_group(int al) { return al >= 4 ? 2 : al == 2 ? 1 : 0; }
+// task group of an LDS projector variant: al = 1, 2, 4, FPV_CELL -> 0..3
+inline int fl_group(int al) { return al == FPV_CELL ? 3 : al >= 4 ? 2 : al == 2 ? 1 : 0; }
+constexpr int FL_GROUPS = 4;
 
 }  // namespace
 }  // namespace acu
@@ -219,7 +221,7 @@ struct acu_batch {
   int ff_total = 0, ff_maxL = 0;  // FFT ramp blocks and the largest transform
   int rf_total = 0, bp_total = 0, cl_total = 0, fp_total = 0, fs_total = 0, ew_total = 0,
       sw_total = 0;
-  int fl_total[3] = {0, 0, 0};  // LDS projector blocks per angles-per-lane group (1, 2, 4)
+  int fl_total[4] = {0, 0, 0, 0};  // LDS projector blocks per variant group (lds1, lds2, lds4, cell16)
   int ramp_variant = acu::RAMP_FP64;  // arithmetic of the ramp filter
   // task sets for the fixed operand combinations of FBP / TV
   acu::RFSet rf_f, rf_fk, rf_ruk, rf_rp;  // ramp filter of f, fk, ruk, rp
@@ -366,8 +368,9 @@ FPLTask fpl_task(const Slice& s, const float* img, float* out) {
     const char* e = std::getenv("ACUMEN_FPL_PITCH");
     return e && std::strcmp(e, "fixed") == 0;
   }();
-  return FPLTask{s.N, s.T, s.mid, s.fl_begin, s.fpv, (s.T + 4 * s.fpv - 1) / (4 * s.fpv),
-                 img, s.cells, s.scfp, out, fixed_pitch ? nullptr : s.pcls[fl_group(s.fpv)]};
+  const int blk = fpl_angle_block(s.fpv);
+  const uint8_t* pc = (fixed_pitch || s.fpv == FPV_CELL) ? nullptr : s.pcls[fl_group(s.fpv)];
+  return FPLTask{s.N, s.T, s.mid, s.fl_begin, s.fpv, (s.T + blk - 1) / blk, img, s.cells, s.scfp, out, pc};
 }
 CellTask cell_task(const Slice& s, const float* img) {
   return CellTask{s.N, s.cl_begin, img, s.cells, s.cellsT};
@@ -389,7 +392,7 @@ void build(acu_batch* b, DevArray<T>& arr, F&& make) {
 // within the slice's angles-per-lane group, gather blocks within the
 // gather group.
 void assign_fp(acu_batch* b) {
-  int cl = 0, fp = 0, fs = 0, fl[3] = {0, 0, 0};
+  int cl = 0, fp = 0, fs = 0, fl[FL_GROUPS] = {0, 0, 0, 0};
   for (auto& s : b->s) {
     s.fpl_cells = s.fpv == 0 || !s.theta_set || fpl_may_overflow(s.N, s.T, s.theta.data(), s.fpv);
     s.cl_begin = cl; s.cl_n = s.fpl_cells ? cell_blocks(s.N) : 0; cl += s.cl_n;
@@ -403,13 +406,13 @@ void assign_fp(acu_batch* b) {
     }
   }
   b->cl_total = cl; b->fp_total = fp; b->fs_total = fs;
-  for (int g = 0; g < 3; ++g) b->fl_total[g] = fl[g];
+  for (int g = 0; g < FL_GROUPS; ++g) b->fl_total[g] = fl[g];
 }
 
 template <class ImgF, class OutF>
 void build_fpset(acu_batch* b, FPSet& set, ImgF img, OutF out) {
   std::vector<CellTask> cl;
-  std::vector<FPLTask> fl[3];
+  std::vector<FPLTask> fl[FL_GROUPS];
   std::vector<FPTask> fp;
   for (auto& s : b->s) {
     if (s.cl_n > 0) cl.push_back(cell_task(s, img(s)));
@@ -417,7 +420,7 @@ void build_fpset(acu_batch* b, FPSet& set, ImgF img, OutF out) {
     else fp.push_back(fp_task(s, out(s)));
   }
   set.cl.upload(cl);
-  for (int g = 0; g < 3; ++g) set.fl[g].upload(fl[g]);
+  for (int g = 0; g < FL_GROUPS; ++g) set.fl[g].upload(fl[g]);
   set.fp.upload(fp);
 }
 
@@ -525,11 +528,11 @@ void iradon_seq(acu_batch* b, const RFSet& rf, const DevArray<BPTask>& bp, int s
 // per non-empty variant group.
 void radon_seq(acu_batch* b, const FPSet& set, int sl) {
   acu_ctx* c = b->ctx;
-  static const int al_of[3] = {1, 2, 4};
+  static const int al_of[FL_GROUPS] = {1, 2, 4, FPV_CELL};
   if (sl < 0) {
     if (b->cl_total > 0)
       timed(c, K_CELLS, [&] { launch_cells(set.cl.d, set.cl.n, 0, b->cl_total, c->stream); });
-    for (int g = 0; g < 3; ++g)
+    for (int g = 0; g < FL_GROUPS; ++g)
       if (b->fl_total[g] > 0)
         timed(c, K_FP, [&] {
           launch_forward_lds(set.fl[g].d, set.fl[g].n, 0, b->fl_total[g], al_of[g], c->stream);
@@ -790,7 +793,9 @@ double time_variant(acu_batch* b, int sl, int v, int reps) {
   return (double)ms / reps;
 }
 
-const char* variant_name(int v) { return v == 0 ? "gather" : v == 1 ? "lds1" : v == 2 ? "lds2" : "lds4"; }
+const char* variant_name(int v) {
+  return v == 0 ? "gather" : v == 1 ? "lds1" : v == 2 ? "lds2" : v == FPV_CELL ? "cell16" : "lds4";
+}
 
 const char* ramp_name(int v) {
   return v == RAMP_FP64 ? "fp64" : v == RAMP_FP32 ? "fp32" : v == RAMP_BF16X3 ? "bf16x3"
@@ -824,7 +829,8 @@ double time_ramp(acu_batch* b, int v, int reps) {
 std::string autotune(acu_batch* b, int reps) {
   require_theta(b);
   b->ctx->sync();
-  static const int cand[4] = {1, 2, 4, 0};
+  static const int cand[5] = {1, 2, 4, FPV_CELL, 0};
+  constexpr int NC = 5;
   std::map<std::tuple<int, int, uint64_t>, std::pair<int, std::vector<double>>> seen;
   std::ostringstream js;
   js << "{\"reps\":" << reps << ",\"slices\":[";
@@ -833,9 +839,9 @@ std::string autotune(acu_batch* b, int reps) {
     const auto key = std::make_tuple(s.N, s.T, theta_hash(s.theta));
     auto it = seen.find(key);
     if (it == seen.end()) {
-      std::vector<double> ms(4);
+      std::vector<double> ms(NC);
       int best = 0;
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < NC; ++k) {
         ms[k] = time_variant(b, (int)i, cand[k], reps);
         if (ms[k] < ms[best]) best = k;
       }
@@ -843,7 +849,7 @@ std::string autotune(acu_batch* b, int reps) {
     }
     s.fpv = it->second.first;
     js << (i ? "," : "") << "{\"slice\":" << i << ",\"N\":" << s.N << ",\"T\":" << s.T << ",\"ms\":{";
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < NC; ++k)
       js << (k ? "," : "") << "\"" << variant_name(cand[k]) << "\":" << it->second.second[k];
     js << "},\"choice\":\"" << variant_name(s.fpv) << "\"}";
   }
@@ -1082,7 +1088,7 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       s.bp_begin = bp; s.bp_n = s.ntx * s.nty; bp += s.bp_n;
       s.fp_clen = fp_clen_env > 0 ? fp_clen_env : acu::FP_CHUNK_DEFAULT;
       s.fpv = gather ? 0
-                     : (al_env == 1 || al_env == 2 || al_env == 4) ? al_env
+                     : (al_env == 1 || al_env == 2 || al_env == 4 || al_env == acu::FPV_CELL) ? al_env
                                                                    : acu::fpl_angles_per_lane(s.N, s.T);
       s.ew_begin = ew; s.ew_n = acu::ew_blocks((size_t)s.N * s.N); ew += s.ew_n;
       s.sw_begin = sw; s.sw_n = acu::ew_blocks((size_t)s.N * s.T); sw += s.sw_n;
@@ -1308,8 +1314,8 @@ int acu_batch_fp_variant(const acu_batch* b, int slice) {
 int acu_batch_set_fp_variant(acu_batch* b, int slice, int variant) {
   if (!b || slice < 0 || slice >= (int)b->s.size())
     return fail(ACU_ERR_ARG, "acu_batch_set_fp_variant: bad slice");
-  if (variant != 0 && variant != 1 && variant != 2 && variant != 4)
-    return fail(ACU_ERR_ARG, "acu_batch_set_fp_variant: variant must be 0, 1, 2 or 4");
+  if (variant != 0 && variant != 1 && variant != 2 && variant != 4 && variant != acu::FPV_CELL)
+    return fail(ACU_ERR_ARG, "acu_batch_set_fp_variant: variant must be 0, 1, 2, 4 or 8");
   ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync(); b->s[slice].fpv = variant; acu::refresh_fp(b));
 }
 

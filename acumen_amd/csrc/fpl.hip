@@ -26,7 +26,7 @@
 //    bounds test is applied, so every edge decision is the reference's own;
 //  * a chunk whose box does not fit the LDS budget falls back to gathering
 //    the bilinear cells of fp.hip from global memory (exact positions).
-#include "acu_internal.hpp"
+#include "fpl_common.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -40,37 +40,6 @@ constexpr int FPL_THREADS = 256;
 #endif
 constexpr int AMAX = ACUMEN_FPL_AMAX;  // max angles per lane (1, 2 or 4)
 constexpr int FPL_SB = 8;  // staging loads in flight per lane
-
-__device__ __forceinline__ int find_task(const FPLTask* tasks, int ntasks, int gblock) {
-  int lo = 0, hi = ntasks - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (tasks[mid].block_begin <= gblock) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ float wmin(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-  return v;
-}
-__device__ __forceinline__ float wmax(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-  return v;
-}
-
-// Range of step offsets u = x - m over which a + b*u stays in [lo, hi].
-__device__ __forceinline__ void urange(float a, float b, float lo, float hi, float& u0, float& u1) {
-  if (fabsf(b) < 1e-6f) {
-    if (a >= lo && a <= hi) { u0 = -1e30f; u1 = 1e30f; } else { u0 = 1e30f; u1 = -1e30f; }
-  } else {
-    const float p = (lo - a) / b, q = (hi - a) / b;
-    u0 = fminf(p, q);
-    u1 = fmaxf(p, q);
-  }
-}
 
 // One workgroup's work; PITCH is the LDS row pitch of its boxes (odd, so
 // that the taps of 64 neighbouring rays spread over the banks; which odd
@@ -103,15 +72,15 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
     acc[j] = 0.0f;
     if (act[j]) {
       float ua, ub, va, vb;
-      urange(m - sy[j], c[j], 1.99f, (float)N + 0.01f, ua, ub);
-      urange(m + cyv[j], s[j], 1.99f, (float)N + 0.01f, va, vb);
+      fpl_urange(m - sy[j], c[j], 1.99f, (float)N + 0.01f, ua, ub);
+      fpl_urange(m + cyv[j], s[j], 1.99f, (float)N + 0.01f, va, vb);
       const float u0 = fmaxf(ua, va), u1 = fminf(ub, vb);
       if (u0 <= u1) { lu0 = fminf(lu0, u0); lu1 = fmaxf(lu1, u1); }
     }
   }
   // workgroup step window (1-based steps, one step of margin)
-  lu0 = wmin(lu0);
-  lu1 = wmax(lu1);
+  lu0 = fpl_wmin(lu0);
+  lu1 = fpl_wmax(lu1);
   if (lane == 0) { red[wave][0] = lu0; red[wave][1] = lu1; }
   __syncthreads();
   const float U0 = fminf(fminf(red[0][0], red[1][0]), fminf(red[2][0], red[3][0]));
@@ -124,32 +93,13 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
   }
 
   const ACU_GLOBAL float* __restrict__ img = as_global(tk.img);
+  const FplBoxGeom bg = fpl_box_geom(tk, ab * 4 * AL, 4 * AL, rb * 64, min(N - 1, rb * 64 + 63), lane);
   for (int c0 = XS; c0 <= XE; c0 += FPL_U) {
     const int c1 = min(XE, c0 + FPL_U - 1);
-    // bounding box of the chunk's sample positions (fast form, affine in the
-    // step: extremes at c0 and c1)
-    float mnx = 1e30f, mny = 1e30f, mxx = -1e30f, mxy = -1e30f;
-    const float ua = (float)c0 - m, ub = (float)c1 - m;
-#pragma unroll
-    for (int j = 0; j < AL; ++j) {
-      if (!act[j]) continue;
-      const float xa = fmaf(c[j], ua, m - sy[j]), xb = fmaf(c[j], ub, m - sy[j]);
-      const float ya = fmaf(s[j], ua, m + cyv[j]), yb = fmaf(s[j], ub, m + cyv[j]);
-      mnx = fminf(mnx, fminf(xa, xb)); mxx = fmaxf(mxx, fmaxf(xa, xb));
-      mny = fminf(mny, fminf(ya, yb)); mxy = fmaxf(mxy, fmaxf(ya, yb));
-    }
-    mnx = wmin(mnx); mny = wmin(mny); mxx = wmax(mxx); mxy = wmax(mxy);
-    if (lane == 0) { red[wave][0] = mnx; red[wave][1] = mny; red[wave][2] = mxx; red[wave][3] = mxy; }
-    __syncthreads();
-    mnx = fminf(fminf(red[0][0], red[1][0]), fminf(red[2][0], red[3][0]));
-    mny = fminf(fminf(red[0][1], red[1][1]), fminf(red[2][1], red[3][1]));
-    mxx = fmaxf(fmaxf(red[0][2], red[1][2]), fmaxf(red[2][2], red[3][2]));
-    mxy = fmaxf(fmaxf(red[0][3], red[1][3]), fmaxf(red[2][3], red[3][3]));
-    __syncthreads();
-    if (!(mnx <= mxx)) continue;  // no active lane in this workgroup (uniform)
-    // 1-based pixel box with two pixels of margin on every side
-    const int bx0 = (int)floorf(fmaxf(mnx, -1e6f)) - 2, by0 = (int)floorf(fmaxf(mny, -1e6f)) - 2;
-    const int bw = (int)floorf(fminf(mxx, 1e6f)) - bx0 + 4, bh = (int)floorf(fminf(mxy, 1e6f)) - by0 + 4;
+    // bounding box of the chunk's samples (fpl_box: wave-uniform, no LDS)
+    const FplBox B = fpl_box(bg, m, c0, c1);
+    if (!B.any) continue;  // no active lane in this workgroup (uniform)
+    const int bx0 = B.x0, by0 = B.y0, bw = B.w, bh = B.h;
     const bool fits = bw > 0 && bh > 0 && bw <= PITCH && bh * PITCH <= FPL_CAP;
     if (fits) {
       // row staging: lanes cover 128 columns, the wave pair (wave >> 1) picks
@@ -272,7 +222,7 @@ __global__ __launch_bounds__(FPL_THREADS) void fpl_kernel(const FPLTask* __restr
   const int lbk = xcd_remap(blockIdx.x, gridDim.x);
   if (lbk >= nblocks) return;
   const int gb = base + lbk;
-  const FPLTask& tk = tasks[find_task(tasks, ntasks, gb)];
+  const FPLTask& tk = tasks[fpl_find_task(tasks, ntasks, gb)];
   const int ab = (gb - tk.block_begin) % tk.nab;
   switch (tk.pclass ? (int)tk.pclass[ab] : FPL_PCLASS_DEFAULT) {
     case 0: fpl_body<AL, FPL_PITCHES[0]>(tk, gb, box, red); break;
@@ -367,6 +317,7 @@ void fpl_pitch_classes(int N, int T, const float* theta_deg, int al, uint8_t* ou
 }
 
 bool fpl_may_overflow(int N, int T, const float* theta_deg, int al) {
+  if (al == FPV_CELL) return fplc_may_overflow(N, T, theta_deg);
   // The box of a chunk spans (per axis) at most the 64-ray x FPL_U-step
   // footprint of one angle, hypot(63, FPL_U - 1), plus the drift of a
   // sample between the first and last angle of its block, span * r with r
@@ -392,6 +343,7 @@ bool fpl_may_overflow(int N, int T, const float* theta_deg, int al) {
 }
 
 int fpl_blocks(int N, int T, int al) {
+  if (al == FPV_CELL) return fplc_blocks(N, T);
   const int nrb = (N + 63) / 64;
   const int nab = (T + 4 * al - 1) / (4 * al);
   return nrb * nab;
@@ -400,6 +352,7 @@ int fpl_blocks(int N, int T, int al) {
 void launch_forward_lds(const FPLTask* d_tasks, int ntasks, int base, int blocks, int al,
                         hipStream_t s) {
   if (blocks <= 0) return;
+  if (al == FPV_CELL) { launch_forward_cell(d_tasks, ntasks, base, blocks, s); return; }
   const dim3 g(xcd_grid(blocks)), t(FPL_THREADS);
   if (al >= 4)
     hipLaunchKernelGGL(fpl_kernel<4>, g, t, 0, s, d_tasks, ntasks, base, blocks);

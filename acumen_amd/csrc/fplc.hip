@@ -1,0 +1,264 @@
+// fplc.hip -- forward projection (radon, tomography/tv_hpc.F90:129-159) with
+// the image staged in LDS as bilinear cells.
+//
+// Same definition, blocks, chunks and boxes as fpl.hip with 4 angles per
+// lane: one workgroup = 64 neighbouring rays x 16 neighbouring angles, the
+// sweep in chunks of FPL_U steps, a per-chunk bounding box of the samples
+// with two pixels of margin.  What differs is the layout of the box and the
+// shape of the workgroup:
+//
+//  * every LDS element is the 2x2 bilinear cell of its pixel, stored as
+//    (v00, v10 - v00, v01, v11 - v01), 16 bytes, so a sample is one
+//    ds_read_b128 instead of two ds_read2_b32.  On gfx950 a 16-byte read
+//    per lane at the projector's scattered addresses moves 1.2-2.5x the
+//    data per clock of the two 8-byte reads (measured).  The differences
+//    are formed once per staged pixel, so a sample needs two FMAs for the
+//    rows instead of two subtractions and two FMAs, and the result is the
+//    same FP32 value as fpl.hip's: top = fma(fx, v10 - v00, v00),
+//    bot = fma(fx, v11 - v01, v01), sample = fma(fy, bot - top, top);
+//  * a cell box of FPLC_PITCH x FPLC_ROWS cells takes 158 KB, so a
+//    workgroup is 16 waves (one angle each) and owns the CU's LDS: the
+//    occupancy (16 waves per CU) is that of four 4-wave fpl workgroups.
+//
+// Staging: the 1024 threads cover 2 column groups of 63 cells (a lane also
+// loads the next column for its left neighbour) x 8 row groups of 13 cell
+// rows (14 pixel rows); a pixel is loaded once, its right neighbour comes
+// from the next lane, its lower neighbour from the lane's next row.
+//
+// Interior chunks (every cell of the box passes the bounds test) use
+// positions by FMA from the ray's start; border chunks use the reference's
+// FP32 positions and bounds test; chunks whose box exceeds the LDS read the
+// global cells of fp.hip, exactly as fpl.hip does.
+#include "fpl_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace acu {
+namespace {
+
+constexpr int C_WAVES = 16;
+constexpr int C_THREADS = 64 * C_WAVES;
+constexpr int C_COLS = 63;                 // cells per column group
+constexpr int C_RG = 13;                   // cell rows per row group
+constexpr int C_LDS = FPLC_CAP * 16 + C_WAVES * 16;  // dynamic LDS bytes: box + reduction scratch
+static_assert(2 * C_COLS >= FPLC_PITCH, "two column groups cover a box row");
+static_assert((C_WAVES / 2) * C_RG >= FPLC_ROWS, "eight row groups cover the box");
+
+// the value of the next lane (lane 63 keeps its own)
+__device__ __forceinline__ float lane_next(float v) { return fpl_dpp<0x130>(v); }
+
+typedef float cellv __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const cellv lds_cellv;
+// the cell at index i of the box whose LDS byte address is lbase: one
+// shift-add on top of the cell index's multiply-add, one ds_read_b128
+__device__ __forceinline__ float4 cell_at(unsigned lbase, unsigned i) {
+  const cellv q = *reinterpret_cast<lds_cellv*>((size_t)(lbase + (i << 4)));
+  return make_float4(q.x, q.y, q.z, q.w);
+}
+
+__device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __restrict__ box,
+                                          float (*red)[4]) {
+#pragma clang fp contract(off)
+  const int N = tk.N, T = tk.T;
+  const float m = tk.mid;
+  const int local = gb - tk.block_begin;
+  const int rb = local / tk.nab, ab = local - rb * tk.nab;  // ray-block-major
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ray = rb * 64 + lane;
+  const float ym = (float)(min(ray, N - 1) + 1) - m;
+  const int ang = ab * C_WAVES + wave;
+  const bool act = ang < T && ray < N;
+  const float2 scv = const_f2(tk.sc, min(ang, T - 1));
+  const float s = scv.x, c = scv.y;
+  const float sy = s * ym, cyv = c * ym;  // one rounding each (tv_hpc.F90:143-144)
+  float acc = 0.0f;
+
+  // workgroup step window (1-based steps, one step of margin)
+  float lu0 = 1e30f, lu1 = -1e30f;
+  if (act) {
+    float ua, ub, va, vb;
+    fpl_urange(m - sy, c, 1.99f, (float)N + 0.01f, ua, ub);
+    fpl_urange(m + cyv, s, 1.99f, (float)N + 0.01f, va, vb);
+    const float u0 = fmaxf(ua, va), u1 = fminf(ub, vb);
+    if (u0 <= u1) { lu0 = u0; lu1 = u1; }
+  }
+  lu0 = fpl_wmin(lu0);
+  lu1 = fpl_wmax(lu1);
+  if (lane == 0) { red[wave][0] = lu0; red[wave][1] = lu1; }
+  __syncthreads();
+  float U0 = red[0][0], U1 = red[0][1];
+#pragma unroll
+  for (int w = 1; w < C_WAVES; ++w) { U0 = fminf(U0, red[w][0]); U1 = fmaxf(U1, red[w][1]); }
+  __syncthreads();
+  int XS = 1, XE = 0;
+  if (U0 <= U1) {
+    XS = max(1, (int)floorf(fmaxf(U0 + m, -1e9f)) - 1);
+    XE = min(N, (int)ceilf(fminf(U1 + m, 1e9f)) + 1);
+  }
+
+  const ACU_GLOBAL float* __restrict__ img = as_global(tk.img);
+  // staging coordinates of this thread: column group, lane column, row group
+  const int scol = (wave & 1) * C_COLS + lane;
+  const int srow0 = (wave >> 1) * C_RG;
+
+  // the workgroup's box of the samples of steps [k0, min(XE, k0 + FPL_U - 1)]
+  // (computed by every wave, see fpl_box), and whether it fits the LDS
+  struct Box { int x0, y0, w, h; bool any, fits; };
+  const FplBoxGeom bg = fpl_box_geom(tk, ab * C_WAVES, C_WAVES, rb * 64, min(N - 1, rb * 64 + 63), lane);
+  auto box_of = [&](int k0) {
+    const FplBox b = fpl_box(bg, m, k0, min(XE, k0 + FPL_U - 1));
+    Box bx{b.x0, b.y0, b.w, b.h, b.any, false};
+    bx.fits = b.any && b.w > 0 && b.h > 0 && b.w <= FPLC_PITCH && b.h <= FPLC_ROWS;
+    return bx;
+  };
+  // This thread's pixels of a box: column scol (and its right neighbour
+  // through the next lane), rows srow0 .. srow0 + C_RG; zero outside the image.
+  float v[C_RG + 1];
+  auto load_pixels = [&](const Box& bx) {
+    const int gx = bx.x0 + scol;
+    const bool xin = bx.fits && scol <= bx.w && gx >= 1 && gx <= N;
+#pragma unroll
+    for (int k = 0; k <= C_RG; ++k) {
+      const int gy = bx.y0 + srow0 + k;
+      v[k] = 0.0f;
+      if (xin && srow0 + k <= bx.h && gy >= 1 && gy <= N) v[k] = img[(size_t)(gy - 1) * N + (gx - 1)];
+    }
+  };
+
+  // Software pipeline: the pixels of chunk k+1 are loaded into registers
+  // while chunk k is sampled from LDS, and written as cells after the
+  // barrier that ends chunk k.
+  Box cur = box_of(XS);
+  load_pixels(cur);
+  for (int c0 = XS; c0 <= XE; c0 += FPL_U) {
+    const int c1 = min(XE, c0 + FPL_U - 1);
+    // (1) this chunk's cells from the registers (every wave passed the
+    // barrier that ended the previous chunk)
+    if (cur.fits) {
+      float nb[C_RG + 1];
+#pragma unroll
+      for (int k = 0; k <= C_RG; ++k) nb[k] = lane_next(v[k]);  // the pixel to the right
+      const bool cw = lane < C_COLS && scol < cur.w;
+#pragma unroll
+      for (int k = 0; k < C_RG; ++k)
+        if (cw && srow0 + k < cur.h)
+          box[(srow0 + k) * FPLC_PITCH + scol] = make_float4(v[k], nb[k] - v[k], v[k + 1], nb[k + 1] - v[k + 1]);
+    }
+    __syncthreads();
+    // (2) the next chunk's box, and its pixels in flight during (3)
+    const Box nxt = box_of(c0 + FPL_U);
+    load_pixels(nxt);
+    // (3) sample this chunk
+    const int bx0 = cur.x0, by0 = cur.y0, bw = cur.w, bh = cur.h;
+    if (cur.fits) {
+      const bool interior = bx0 >= 2 && by0 >= 2 && bx0 + bw - 1 <= N && by0 + bh - 1 <= N;
+      if (interior) {
+        // every cell in the box passes the bounds test: fast positions, no
+        // branches; an inactive (ray, angle) reads a fixed in-box cell and
+        // its sum is never stored
+        const float cj = act ? c : 0.0f, sj = act ? s : 0.0f;
+        const float bxl = act ? (m - sy) - (float)bx0 : 1.0f;
+        const float byl = act ? (m + cyv) - (float)by0 : 1.0f;
+        const unsigned lbase = (unsigned)(uintptr_t)box;  // LDS byte address of the box (uniform)
+        float u = (float)c0 - m;  // integer-valued, so u += 1 is exact
+#pragma unroll 4
+        for (int x = c0; x <= c1; ++x, u += 1.0f) {
+          const float xl = fmaf(cj, u, bxl);
+          const float yl = fmaf(sj, u, byl);
+          const int ix = (int)xl, iy = (int)yl;
+          const float fx = __builtin_amdgcn_fractf(xl), fy = __builtin_amdgcn_fractf(yl);
+          const float4 q = cell_at(lbase, __umul24((unsigned)iy, FPLC_PITCH) + (unsigned)ix);
+          const float top = fmaf(fx, q.y, q.x);
+          const float bot = fmaf(fx, q.w, q.z);
+          acc += fmaf(fy, bot - top, top);
+        }
+      } else if (act) {
+        // border chunk: the reference's positions and bounds test
+        for (int x = c0; x <= c1; ++x) {
+          const float u = (float)x - m;
+          const float xr = (c * u - sy) + m;
+          const float yr = (s * u + cyv) + m;
+          const float fxr = floorf(xr), fyr = floorf(yr);
+          const int x1 = (int)fxr, y1 = (int)fyr;
+          if (x1 >= 2 && x1 <= N - 1 && y1 >= 2 && y1 <= N - 1) {
+            const float fx = xr - fxr, fy = yr - fyr;
+            const float4 q = box[__mul24(y1 - by0, FPLC_PITCH) + (x1 - bx0)];
+            const float top = fmaf(fx, q.y, q.x);
+            const float bot = fmaf(fx, q.w, q.z);
+            acc += fmaf(fy, bot - top, top);
+          }
+        }
+      }
+    } else if (cur.any && act) {
+      // box too large for LDS: bilinear cells from global memory (as fpl.hip)
+      const int P = N + 1;
+      const float fN = (float)N;
+      for (int x = c0; x <= c1; ++x) {
+        const float u = (float)x - m;
+        const float xr = (c * u - sy) + m;
+        const float yr = (s * u + cyv) + m;
+        const int cx = (int)__builtin_amdgcn_fmed3f(xr, 0.0f, fN);
+        const int cyi = (int)__builtin_amdgcn_fmed3f(yr, 0.0f, fN);
+        const float wx = __builtin_amdgcn_fractf(xr), wy = __builtin_amdgcn_fractf(yr);
+        const float4 q = global_f4(as_global(tk.cells), (unsigned)(__mul24(cyi, P) + cx));
+        const float t1 = fmaf(wx, q.w, q.z);
+        const float t2 = fmaf(wx, q.y, q.x);
+        acc += fmaf(wy, t1, t2);
+      }
+    }
+    __syncthreads();  // every wave is done with this chunk's cells
+    cur = nxt;
+  }
+  if (act) tk.sino[(size_t)ang * N + ray] = acc / (float)N;
+}
+
+__global__ __launch_bounds__(C_THREADS) void fplc_kernel(const FPLTask* __restrict__ tasks, int ntasks, int base,
+                                                          int nblocks) {
+  // dynamic LDS: the cell box at address 0 (so a cell's address is one
+  // multiply-add), then per-wave min x, min y, max x, max y (or step range)
+  extern __shared__ float4 cbox[];
+  float (*red)[4] = reinterpret_cast<float (*)[4]>(cbox + FPLC_CAP);
+  const int lbk = xcd_remap(blockIdx.x, gridDim.x);
+  if (lbk >= nblocks) return;
+  const int gb = base + lbk;
+  fplc_body(tasks[fpl_find_task(tasks, ntasks, gb)], gb, cbox, red);
+}
+
+}  // namespace
+
+bool fplc_may_overflow(int N, int T, const float* theta_deg) {
+  // as fpl_may_overflow for 16-angle blocks, against the cell box
+  const double pi = 3.141592653589793;
+  const double r = 0.70711 * ((double)N + 2.0);
+  const double foot = std::hypot(63.0, (double)(FPL_U - 1));
+  double span = 0.0;
+  for (int a0 = 0; a0 < T; a0 += C_WAVES) {
+    double lo = 1e300, hi = -1e300;
+    for (int a = a0; a < std::min(T, a0 + C_WAVES); ++a) {
+      const double ct = pi * ((double)theta_deg[a] - 90.0) / 180.0;
+      lo = std::min(lo, ct);
+      hi = std::max(hi, ct);
+    }
+    span = std::max(span, hi - lo);
+  }
+  const double extent = foot + (span + 1e-6) * r + 8.0;
+  return extent > (double)std::min(FPLC_PITCH, FPLC_ROWS);
+}
+
+int fplc_blocks(int N, int T) { return ((N + 63) / 64) * ((T + C_WAVES - 1) / C_WAVES); }
+
+void launch_forward_cell(const FPLTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s) {
+  if (blocks <= 0) return;
+  int dev = 0;
+  ACU_HIP_CHECK(hipGetDevice(&dev));
+  static bool attr_set[64] = {};
+  if (dev < 64 && !attr_set[dev]) {
+    ACU_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&fplc_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, C_LDS));
+    attr_set[dev] = true;
+  }
+  hipLaunchKernelGGL(fplc_kernel, dim3(xcd_grid(blocks)), dim3(C_THREADS), C_LDS, s, d_tasks, ntasks, base, blocks);
+}
+
+}  // namespace acu

@@ -182,7 +182,8 @@ class Batch:
         return v
 
     def fp_variant(self, i: int) -> int:
-        """0 = global gathers; 1, 2, 4 = LDS boxes with that many angles per lane."""
+        """0 = global gathers; 1, 2, 4 = LDS pixel boxes with that many angles
+        per lane; 8 = LDS bilinear-cell boxes (16 angles per workgroup)."""
         v = self._lib.acu_batch_fp_variant(self.handle, i)
         if v < 0:
             check(-v)

@@ -141,7 +141,10 @@ int acu_tvopt(acu_batch *b, const float *f_host, float mu, float lambda, int nou
  *
  * Forward-projection variant of a slice: 0 = gathers of bilinear cells
  * from global memory; 1, 2, 4 = LDS-staged pixel boxes with 4, 8 or 16
- * angles per workgroup (1, 2, 4 per lane).  A batch starts from
+ * angles per workgroup (1, 2, 4 per lane); 8 = LDS-staged bilinear-cell
+ * boxes, 16 angles per 1024-thread workgroup (one per wave).  All compute
+ * the same sums; where a chunk's box fits both LDS layouts, 4 and 8 give
+ * identical bits.  A batch starts from
  * ACUMEN_FP_PATH / ACUMEN_FPL_AL or a per-shape rule; acu_batch_autotune
  * times every variant (reps launches each, after one warm-up) for every
  * distinct slice shape (N, T, angles) and keeps the fastest for all later

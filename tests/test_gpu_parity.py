@@ -180,6 +180,9 @@ def test_full_size_tv_against_fp64(ctx):
         b.set_ramp_variant(_lib.RAMP_FFT)
         b.tvopt(0.05, 0.1, 1, 1, 2, ACU_TV_GRAPH)
         results["tuned"] = b.get_image(0, ACU_IMG_TV)
+        b.set_fp_variant(0, 8)  # cell-box projector
+        b.tvopt(0.05, 0.1, 1, 1, 2, ACU_TV_GRAPH)
+        results["cell16"] = b.get_image(0, ACU_IMG_TV)
     ref.set_threads(16)
     try:
         tv64, _, _ = ref.tvopt(f, th, wpu, 0.05, 0.1, 1, 1, 2, "f64")

@@ -12,7 +12,7 @@ from oracle import ref
 pytestmark = pytest.mark.gpu
 
 FBP_TOL, TV_TOL = 1e-5, 1e-4
-VARIANTS = (0, 1, 2, 4)
+VARIANTS = (0, 1, 2, 4, 8)  # gather, LDS boxes with 1/2/4 angles per lane, cell boxes
 
 
 def say(*a):
@@ -33,9 +33,26 @@ def test_every_variant_matches_oracle(ctx, N, T):
             assert rel(got, want) < FBP_TOL, (v, rel(got, want))
 
 
+@pytest.mark.parametrize("N,T", [(257, 1500), (513, 1500), (129, 720)])
+def test_cell_boxes_equal_raw_boxes(ctx, N, T):
+    """The cell-box projector (8) and the raw-box projector with 4 angles per
+    lane (4) share blocks, chunks, boxes and arithmetic: where every box fits
+    both LDS layouts (fine angular steps keep the 16-angle drift small) the
+    sinograms are identical bit for bit."""
+    say("cell16 vs lds4", N, T)
+    th = synth.angles(T)
+    x = disk_phantom(N, 12, 7 * N + T)
+    with Batch(ctx, [(N, T, synth.default_wpu(N), th)]) as b:
+        b.set_fp_variant(0, 4)
+        raw = b.radon(x)
+        b.set_fp_variant(0, 8)
+        cell = b.radon(x)
+    assert np.array_equal(raw.view(np.uint32), cell.view(np.uint32))
+
+
 def test_mixed_variant_batch_equals_single_plans(ctx):
-    shapes = [(65, 48), (97, 60), (129, 37), (65, 100), (161, 24)]
-    variants = [0, 1, 2, 4, 1]
+    shapes = [(65, 48), (97, 60), (129, 37), (65, 100), (161, 24), (97, 33)]
+    variants = [0, 1, 2, 4, 1, 8]
     say("mixed variants", shapes, variants)
     rng = np.random.default_rng(11)
     items = []
@@ -72,7 +89,7 @@ def test_autotune_report_and_results(ctx):
             b.set_sino(i, it[4])
         rep = b.autotune(reps=1)
         assert [s["slice"] for s in rep["slices"]] == list(range(len(shapes)))
-        names = {0: "gather", 1: "lds1", 2: "lds2", 4: "lds4"}
+        names = {0: "gather", 1: "lds1", 2: "lds2", 4: "lds4", 8: "cell16"}
         for i, s in enumerate(rep["slices"]):
             assert (s["N"], s["T"]) == shapes[i]
             assert set(s["ms"]) == set(names.values())
