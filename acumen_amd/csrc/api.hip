@@ -8,6 +8,7 @@
 
 #include <chrono>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <sstream>
 #include <tuple>
@@ -335,6 +336,25 @@ void upload_tables(acu_batch* b, int i) {
   ACU_HIP_CHECK(hipMemcpy(s.tact, act.data(), act.size(), hipMemcpyHostToDevice));
 }
 
+uint64_t theta_hash(const std::vector<float>& th);
+
+// Pitch classes of a slice geometry (host model of the LDS banks, a few tens
+// of ms per geometry): computed once per (N, T, angles, angles per lane) in
+// the process and shared by every slice and batch of that geometry.
+const std::vector<uint8_t>& pitch_classes_cached(int N, int T, const std::vector<float>& theta, int al) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, uint64_t, int>, std::vector<uint8_t>> cache;
+  const auto key = std::make_tuple(N, T, theta_hash(theta), al);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    std::vector<uint8_t> pc((size_t)(T + 4 * al - 1) / (4 * al));
+    fpl_pitch_classes(N, T, theta.data(), al, pc.data());
+    it = cache.emplace(key, std::move(pc)).first;
+  }
+  return it->second;
+}
+
 void upload_theta(acu_batch* b, int i) {
   auto& s = b->s[i];
   std::vector<float2> bp(s.T), fp(s.T);
@@ -343,9 +363,7 @@ void upload_theta(acu_batch* b, int i) {
   ACU_HIP_CHECK(hipMemcpy(s.scbp, bp.data(), bp.size() * sizeof(float2), hipMemcpyHostToDevice));
   ACU_HIP_CHECK(hipMemcpy(s.scfp, fp.data(), fp.size() * sizeof(float2), hipMemcpyHostToDevice));
   for (int g = 0; g < 3; ++g) {
-    const int al = 1 << g;
-    std::vector<uint8_t> pc((size_t)(s.T + 4 * al - 1) / (4 * al));
-    fpl_pitch_classes(s.N, s.T, s.theta.data(), al, pc.data());
+    const std::vector<uint8_t>& pc = pitch_classes_cached(s.N, s.T, s.theta, 1 << g);
     ACU_HIP_CHECK(hipMemcpy(s.pcls[g], pc.data(), pc.size(), hipMemcpyHostToDevice));
   }
   s.theta_set = true;

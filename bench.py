@@ -16,7 +16,9 @@ roofline: the forward projector dominates TV time (6 of the 13 projector
 launches per outer iteration and ~85% of the time); its algorithmic work is
 14 FP32 flops per ray sample (2 coordinates, 2 fractions, 3 lerps, 1
 accumulate) over N*N*T samples per slice, measured with HIP events on the
-library's stream during the timed steps.
+library's stream during the timed steps.  The kernel is bound by its LDS
+gathers rather than the VALU (16 bytes of staged box per sample), so
+`roofline.lds` reports that rate as well.
 
 cpu_baseline: the oracle's FP32 restatement of the reference's loops, one
 slice per host thread as the reference's MPI ranks do, on a bounded sample
@@ -40,6 +42,8 @@ T_ANG = 1500
 MU, LAM, NOUTER, NINNER, NCG = 0.05, 0.1, 50, 1, 5
 FP_FLOPS_PER_SAMPLE = 14
 FP32_PEAK_TFLOPS = 157.3
+LDS_BYTES_PER_SAMPLE = 16  # the 2x2 cell of a bilinear sample
+N_CU, CLOCK_GHZ = 256, 2.4
 
 
 def log(*a):
@@ -62,9 +66,12 @@ def make_inputs(ctx, batch, N, T, seed0):
     return th, wpu, sinos
 
 
-def fp_kernel_name():
-    """The forward-projection kernel the library launches (ACUMEN_FP_PATH)."""
-    return "fp_kernel" if os.environ.get("ACUMEN_FP_PATH") == "gather" else "fpl_kernel"
+def fp_kernel_name(variant=None):
+    """The forward-projection kernel of a projector variant (the autotuner's
+    choice, else ACUMEN_FP_PATH)."""
+    if variant is None:
+        variant = "gather" if os.environ.get("ACUMEN_FP_PATH") == "gather" else "lds4"
+    return {"gather": "fp_kernel", "cell16": "fplc_kernel"}.get(variant, "fpl_kernel")
 
 
 def traffic_from_profile(name=None, key="hbm_bytes_per_launch_per_slice"):
@@ -550,10 +557,12 @@ def main() -> int:
     fp_avg_ms = fp_ms / max(fp_n, 1)
     samples_per_launch = float(N) * N * T * B
     achieved = FP_FLOPS_PER_SAMPLE * samples_per_launch / (fp_avg_ms * 1e-3) / 1e12
-    traffic = traffic_from_profile()
+    fp_name = fp_kernel_name(tuning.get("fp_variant") if tuning else None)
+    traffic = traffic_from_profile(f"{fp_name}_traffic_2049x1500.txt")
+    lds_bytes = LDS_BYTES_PER_SAMPLE * samples_per_launch / (fp_avg_ms * 1e-3)
     roofline = {
         "bound": "mfma",
-        "kernel": f"{fp_kernel_name()} (ray-driven bilinear projection on the FP32 VALU; compute roofline)",
+        "kernel": f"{fp_name} (ray-driven bilinear projection on the FP32 VALU; compute roofline)",
         "achieved": round(achieved, 3),
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
@@ -562,6 +571,9 @@ def main() -> int:
         "avg_launch_ms": round(fp_avg_ms, 4),
         "launches": fp_n,
         "work_per_launch": f"{FP_FLOPS_PER_SAMPLE} flop x {N}*{N}*{T} samples x {B} slices",
+        # the projector is bound by LDS gathers: 16 bytes of box per sample
+        "lds": {"bytes_per_sample": LDS_BYTES_PER_SAMPLE, "achieved_TBps": round(lds_bytes / 1e12, 2),
+                "B_per_clk_per_cu": round(lds_bytes / (N_CU * CLOCK_GHZ * 1e9), 1)},
     }
 
     cpu = None
