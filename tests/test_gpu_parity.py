@@ -215,6 +215,27 @@ def test_tvopt_matches_oracle(ctx, N, T, nouter):
     assert np.allclose(delta, d64, rtol=1e-3)
 
 
+@pytest.mark.parametrize("mode", [ACU_TV_PLAIN, ACU_TV_GRAPH])
+def test_tvopt_active_shrink_and_two_inner(ctx, mode):
+    """lambda = 100 (tomo_run.pbs:21): the soft-threshold 1/lambda = 0.01 is
+    active, unlike at lambda = 0.1; with two inner iterations per outer one.
+    Against the FP64 oracle, and the shrink visibly changes the result."""
+    N, T = 97, 72
+    say("tvopt lambda=100 ninner=2", N, T, mode)
+    th = synth.angles(T)
+    wpu = synth.default_wpu(N)
+    f = ref.radon(disk_phantom(N, 8, 21), th, "f64").astype(np.float32)
+    with plan(ctx, N, T, th) as b:
+        b.set_sino(0, f)
+        b.tvopt(0.05, 100.0, 3, 2, 4, mode)
+        tv = b.get_image(0, ACU_IMG_TV)
+        b.tvopt(0.05, 1e-3, 3, 2, 4, mode)
+        tv_weak = b.get_image(0, ACU_IMG_TV)
+    tv64, _, _ = ref.tvopt(f, th, wpu, 0.05, 100.0, 3, 2, 4, "f64")
+    assert rel(tv, tv64) < TV_TOL, rel(tv, tv64)
+    assert rel(tv, tv_weak) > 1e-3  # the shrink changed u
+
+
 def test_tvopt_small_config(ctx):
     """cfg0: 256^2 (N=257), 180 angles, I0=1e4, mu=0.05, lambda=0.1, 20 x 5."""
     say("tvopt cfg0 20x5")
