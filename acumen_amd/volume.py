@@ -48,3 +48,122 @@ def reconstruct(ctx, sinos, theta, wpu: int, tv=None, fbp: bool = True, batch: i
         int(batch), 1 if tune else 0, stats.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
     return out_fbp, out_tv, {"ms": float(stats[0]), "slices_per_s": float(stats[1]),
                              "setup_ms": float(stats[2]), "stream_slices_per_s": float(stats[3])}
+
+
+def _write_at(path: str, offset_elems: int, a: np.ndarray) -> None:
+    """FP32 values at an element offset, creating the file without
+    truncating it (the drivers' write_reals_at)."""
+    with open(path, "ab"):
+        pass
+    with open(path, "r+b") as fh:
+        fh.seek(4 * offset_elems)
+        fh.write(np.ascontiguousarray(a, np.float32).tobytes())
+
+
+def reconstruct_dataset(ctx, root: str, exp: str, *, nsub: int = 1, nsub_t: int = 1, wpu: int = 64,
+                        tv=None, batch: int = 16, tune: bool = True, threads: int = 8, log=None):
+    """Every slice of the dataset $ROOT_CT_DIR/$EXP_CT_NAME streamed through
+    the GPU with the file contract of tomography/tv_hpc.F90:392-652 (and of
+    acumen_amd/fortran/tv_hip.F90): header offsets 0-3 of
+    fileinfo_projections.dat, theta.dat, the angle window, radial (nsub) and
+    angular (nsub_t) sub-sampling, odd padding, sinograms{js}.dat read and
+    fbp{js}.dat / fbp_tv{js}.dat (fbp_sub / fbp_sub_tv when sub-sampling)
+    written at the reference's 64-bit element offsets.  Without ``tv`` only
+    the FBP files are written.
+
+    Slices are read and written by a pool of ``threads`` host threads and
+    reconstructed by ``reconstruct`` (two lanes overlapping PCIe, staging
+    and kernels).  Returns the ``reconstruct`` statistics plus the read and
+    write seconds."""
+    import math
+    import time
+    from concurrent.futures import ThreadPoolExecutor
+
+    from . import io as aio
+
+    ds = aio.Dataset(root, exp)
+    hdr = np.fromfile(ds.path("fileinfo_projections.dat"), np.float32, 4)
+    nfiles, nimagef, n_files_sino, n_files_rec = (int(v) for v in hdr)
+    theta = np.fromfile(ds.path("theta.dat"), np.float32, nfiles)
+    js0, je0 = aio.angle_window(theta)  # 0-based
+    ntheta = int(math.floor(np.float32(je0 - js0 + 1) / np.float32(nsub_t)))
+    add = 1 if nimagef % 2 == 0 else 0
+    width = nimagef + add
+    nimage = int(math.floor(np.float32(width - 1) / np.float32(nsub))) + 1
+    if ntheta < 1:
+        raise ValueError("the angle window holds no projection")
+    th = theta[js0 + np.arange(ntheta) * nsub_t]
+    rowlen = width * nfiles
+    sinos = np.empty((nimagef, ntheta, nimage), np.float32)
+
+    def read(job):
+        js, off = aio.slice_location(job, nimagef, n_files_sino, rowlen)
+        a = np.fromfile(ds.path(f"sinograms{js}.dat"), np.float32, rowlen, offset=4 * off)
+        if a.size != rowlen:
+            raise IOError(f"sinograms{js}.dat: short read for slice {job}")
+        a = a.reshape(nfiles, width)
+        sinos[job - 1] = a[js0::nsub_t][:ntheta, ::nsub][:, :nimage]
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        list(ex.map(read, range(1, nimagef + 1)))
+    t_read = time.perf_counter() - t0
+    if log:
+        log(f"{nimagef} slices of {nimage}^2 x {ntheta} angles read in {t_read:.2f} s")
+    fbp, tvi, stats = reconstruct(ctx, sinos, th, wpu, tv=tv, fbp=True, batch=batch, tune=tune)
+    sub = nsub + nsub_t != 2
+    names = ("fbp_sub", "fbp_sub_tv") if sub else ("fbp", "fbp_tv")
+    nn = nimage * nimage
+
+    def write(job):
+        js, off = aio.slice_location(job, nimagef, n_files_rec, nn)
+        _write_at(ds.path(f"{names[0]}{js}.dat"), off, fbp[job - 1])
+        if tvi is not None:
+            _write_at(ds.path(f"{names[1]}{js}.dat"), off, tvi[job - 1])
+
+    t0 = time.perf_counter()
+    # slices of one file are written by one thread (no two writers on a file)
+    by_file = {}
+    for job in range(1, nimagef + 1):
+        by_file.setdefault(aio.slice_location(job, nimagef, n_files_rec, nn)[0], []).append(job)
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        list(ex.map(lambda jobs: [write(j) for j in jobs], by_file.values()))
+    t_write = time.perf_counter() - t0
+    stats.update(read_s=t_read, write_s=t_write, nimage=nimage, ntheta=ntheta, slices=nimagef)
+    return stats
+
+
+def main(argv=None) -> int:
+    """python -m acumen_amd.volume: the reference's TV driver environment
+    (ROOT_CT_DIR, EXP_CT_NAME, Sub_Sample_Radial, Sub_Sample_Theta,
+    Ninner_Loops, Nouter_Loops, Opt_ncg, Opt_lambda, Opt_mu, Radon_wpu);
+    Nouter_Loops = 0 writes the FBP files only."""
+    import argparse
+    import os
+
+    from .tomography import Context
+
+    ap = argparse.ArgumentParser(description="Stream a dataset's slices through FBP (+ TV) on one GPU")
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--no-tune", action="store_true")
+    a = ap.parse_args(argv)
+    env = os.environ
+    nouter = int(env.get("Nouter_Loops", "0"))
+    tv = None
+    if nouter > 0:
+        tv = dict(mu=float(env["Opt_mu"]), lam=float(env["Opt_lambda"]), nouter=nouter,
+                  ninner=int(env.get("Ninner_Loops", "1")), ncg=int(env.get("Opt_ncg", "5")))
+    with Context(0) as ctx:
+        st = reconstruct_dataset(ctx, env["ROOT_CT_DIR"], env["EXP_CT_NAME"],
+                                 nsub=int(env.get("Sub_Sample_Radial", "1")),
+                                 nsub_t=int(env.get("Sub_Sample_Theta", "1")),
+                                 wpu=int(env.get("Radon_wpu", "64")), tv=tv, batch=a.batch,
+                                 tune=not a.no_tune, threads=a.threads, log=print)
+    print(f"{st['slices']} slices of {st['nimage']}^2 x {st['ntheta']} angles: read {st['read_s']:.2f} s, "
+          f"reconstruction {st['ms'] / 1e3:.2f} s ({st['slices_per_s']:.1f} slices/s), write {st['write_s']:.2f} s")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
