@@ -204,6 +204,38 @@ def run_mixed(args) -> int:
     return 0
 
 
+def volume_cpu_baseline(sinos, th, wpu, samples, fbp, variants, threads):
+    """CPU baseline of the volume workload: the oracle's FP32 FBP (the
+    reference's loops) of the sampled slices on `threads` host threads,
+    timed; as the checker, the streamed FBP and every ramp arithmetic's FBP
+    of those slices against the oracle's FP64 FBP."""
+    import numpy as np
+
+    from oracle import ref
+
+    ref.set_threads(threads)
+    rel = lambda a, w: float(np.linalg.norm(a - w) / np.linalg.norm(w))
+    acc, wall = {}, 0.0
+    try:
+        for k in samples:
+            f64 = ref.iradon(sinos[k].astype(np.float64), th, wpu, "f64")
+            t0 = time.time()
+            f32 = ref.iradon(sinos[k], th, wpu, "f32")
+            wall += time.time() - t0
+            row = {"stream_vs_f64": rel(fbp[k], f64), "stream_vs_ref_f32": rel(fbp[k], f32),
+                   "ref_f32_vs_f64": rel(f32, f64)}
+            for (kk, name), img in variants.items():
+                if kk == k:
+                    row[name + "_vs_f64"] = rel(img, f64)
+            acc[str(k)] = {a: float(f"{b:.3e}") for a, b in row.items()}
+    finally:
+        ref.set_threads(1)
+    cpu = {"value": round(len(samples) / wall, 4), "unit": "slices/s", "cores": threads, "kind": "port",
+           "sample": f"FBP (ramp filter + backprojection) of {len(samples)} sampled slices by the oracle's "
+                     f"FP32 loops on {threads} threads, {wall:.1f} s"}
+    return cpu, acc
+
+
 def run_volume(args) -> int:
     """A 2048^2 x 256-slice volume (2048 detectors -> N = 2049, 1500 angles,
     Poisson I0 = 5e3; 16 distinct phantoms cycled) streamed from host memory
@@ -214,7 +246,6 @@ def run_volume(args) -> int:
     import numpy as np
 
     from acumen_amd import Batch, Context, _lib, synth, volume
-    from oracle import ref
 
     N, T = synth.padded_size(N_DET), T_ANG
     th = synth.angles(T)
@@ -235,25 +266,20 @@ def run_volume(args) -> int:
     tvp = dict(mu=MU, lam=LAM, nouter=30, ninner=NINNER, ncg=NCG)
     _, tv, st_tv = volume.reconstruct(ctx, sinos[:args.tv_slices], th, wpu, tv=tvp, batch=8, tune=True)
     assert np.all(np.isfinite(tv))
-    # accuracy of FBP on 4 sampled slices, for every ramp-filter arithmetic
-    log("accuracy: FBP of 4 sampled slices vs the FP64 / FP32 oracle")
-    ref.set_threads(args.cpu_threads)
+    # FBP of 4 sampled slices with every ramp-filter arithmetic (the product)
     names = {_lib.RAMP_FP64: "fp64", _lib.RAMP_FP32: "fp32", _lib.RAMP_BF16X3: "bf16x3",
              _lib.RAMP_BF16X6: "bf16x6", _lib.RAMP_FFT: "fft"}
-    acc = {}
     samples = [0, args.slices // 3, (2 * args.slices) // 3, args.slices - 1]
+    variants = {}
     with Batch(ctx, [(N, T, wpu, th)]) as b:
         for k in samples:
-            f64 = ref.iradon(sinos[k].astype(np.float64), th, wpu, "f64")
-            f32 = ref.iradon(sinos[k], th, wpu, "f32")
-            rel = lambda a, w: float(np.linalg.norm(a - w) / np.linalg.norm(w))
-            row = {"stream_vs_f64": rel(fbp[k], f64), "stream_vs_ref_f32": rel(fbp[k], f32),
-                   "ref_f32_vs_f64": rel(f32, f64)}
             for v, name in names.items():
                 b.set_ramp_variant(v)
-                row[name + "_vs_f64"] = rel(b.iradon(sinos[k]), f64)
-            acc[str(k)] = {kk: float(f"{vv:.3e}") for kk, vv in row.items()}
-    ref.set_threads(1)
+                variants[(k, name)] = b.iradon(sinos[k])
+    cpu, acc = None, None
+    if not args.no_cpu:
+        log("cpu baseline: the oracle's FBP of 4 sampled slices; accuracy against its FP64 form")
+        cpu, acc = volume_cpu_baseline(sinos, th, wpu, samples, fbp, variants, args.cpu_threads)
     ctx.close()
     out = {
         "metric": "FBP slices/s streamed (PCIe incl.), 2048^2 x 1500 x 256-slice volume; FBP+TV(30) slices/s",
@@ -276,20 +302,43 @@ def run_volume(args) -> int:
         "setup_ms": round(st_fbp["setup_ms"], 1),
         "tv30_stream_slices_per_s": round(st_tv["slices_per_s"], 4),
         "fbp_accuracy_sampled": acc,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def prep_cpu_baseline(proj, sino, th, sh, ks):
+    """CPU baseline of the preprocessing workload: the oracle's 3x3x3 median
+    and correction of projections ks (one host thread), timed; as the
+    checker, the GPU's sinogram rows of those projections bit for bit."""
+    import numpy as np
+
+    from oracle import ref
+
+    exact, wall = True, 0.0
+    for k in ks:
+        t0 = time.time()
+        row = ref.prep_correct(ref.prep_median27(proj[k - 1], proj[k], proj[k + 1]), th, sh)
+        wall += time.time() - t0
+        exact = exact and bool(np.array_equal(sino[:, k, :], row))
+    n = proj.shape[1]
+    cpu = {"value": round(len(ks) / wall, 3), "unit": "projections/s", "cores": 1, "kind": "port",
+           "sample": f"median + correction of {len(ks)} projections of {n}^2 by the oracle, one thread, "
+                     f"{wall:.1f} s"}
+    return cpu, exact
 
 
 def run_prep(args) -> int:
     """Projection -> sinogram preprocessing at 2048^2 projections: the
     centre-of-rotation / tilt search on the 0 and 180 degree projections,
     and the median filter + correction of 64 projections into sinogram rows
-    (host -> GPU -> host, PCIe included), against the oracle on a sample."""
+    (host -> GPU -> host, PCIe included).  cpu_baseline: the oracle's
+    median + correction of a sample of projections on one host thread, which
+    also checks the GPU's sinogram rows of that sample bit for bit."""
     import numpy as np
 
     from acumen_amd import Context, prep, synth
-    from oracle import ref
 
     n, count = N_DET, 64
     ctx = Context(0)
@@ -312,10 +361,11 @@ def run_prep(args) -> int:
     t = time.perf_counter()
     sino = prep.sinograms(ctx, proj, th, sh)
     t_pipe = time.perf_counter() - t
-    k = count // 2
-    med = ref.prep_median27(proj[k - 1], proj[k], proj[k + 1])
-    exact = bool(np.array_equal(sino[:, k, :], ref.prep_correct(med, th, sh)))
     ctx.close()
+    cpu, exact = None, None
+    if not args.no_cpu:
+        log("cpu baseline: the oracle's median + correction of 4 projections")
+        cpu, exact = prep_cpu_baseline(proj, sino, th, sh, [1, count // 3, count // 2, count - 2])
     out = {
         "metric": "projections/s through median filter + correction into sinogram rows (PCIe incl.), 2048^2",
         "value": round(count / t_pipe, 2),
@@ -336,6 +386,7 @@ def run_prep(args) -> int:
         "found_shift_px": round(sh * n, 4),
         "found_theta_rad": th,
         "bit_exact_vs_oracle_sample": exact,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -353,7 +404,6 @@ def run_ingest(args) -> int:
     import torch
 
     from acumen_amd import Context, _lib, ingest
-    from oracle import ref
 
     n, count = N_DET, 256
     npix = n * n
@@ -387,12 +437,7 @@ def run_ingest(args) -> int:
     ctx.set_timing(False)
     avg_s = k_ms / max(k_n, 1) / 1e3
     alg_bytes = 8.0 * npix * count + 4.0 * npix
-    # parity sample against the oracle (2 projections)
-    got = out[:2].cpu().numpy()
-    want = ref.ingest_normalize(imgs[:2].cpu().numpy(), ob.cpu().numpy())
-    gi = got.view(np.int32).astype(np.int64)
-    wi = want.view(np.int32).astype(np.int64)
-    ulps = int(np.abs(gi - wi).max())
+    got16 = out[:16].cpu().numpy()
     # host-buffer form (PCIe both ways), 32 projections
     h_imgs, h_ob = imgs[:32].cpu().numpy(), ob.cpu().numpy()
     norm = ingest.Normalizer(ctx)
@@ -400,12 +445,16 @@ def run_ingest(args) -> int:
     t = time.perf_counter()
     norm.normalize(h_imgs, h_ob)
     pcie = 32 / (time.perf_counter() - t)
-    cpu = None
+    cpu, ulps = None, None
     if not args.no_cpu:
         log("cpu baseline: oracle normalisation, one thread")
+        from oracle import ref
         done, t = 0, time.perf_counter()
         while done < 256 and time.perf_counter() - t < 10.0:
-            ref.ingest_normalize(h_imgs[done % 32:done % 32 + 16], h_ob)
+            want = ref.ingest_normalize(h_imgs[done % 32:done % 32 + 16], h_ob)
+            if done == 0:  # the checker: the GPU's first 16 projections, in ulps
+                gi = got16.view(np.int32).astype(np.int64)
+                ulps = int(np.abs(gi - want.view(np.int32).astype(np.int64)).max())
             done += 16
         el = time.perf_counter() - t
         cpu = {"value": round(done / el, 2), "unit": "projections/s", "cores": 1, "kind": "port",
