@@ -41,10 +41,28 @@ constexpr int C_WAVES = 16;
 constexpr int C_THREADS = 64 * C_WAVES;
 constexpr int C_COLS = 63;                 // cells per column group
 constexpr int C_RG = 13;                   // cell rows per row group
-constexpr int C_LDS = FPLC_CAP * 16 + C_WAVES * 16;  // dynamic LDS bytes: box + reduction scratch
+constexpr int C_TAB = C_THREADS / 16;     // chunk boxes per table (16 threads per chunk)
+constexpr int C_LDS = FPLC_CAP * 16 + C_WAVES * 16 + C_TAB * 16;  // dynamic LDS: box, scratch, table
+static_assert(C_LDS <= 163840, "LDS of one CU");
 static_assert(2 * C_COLS >= FPLC_PITCH, "two column groups cover a box row");
 static_assert((C_WAVES / 2) * C_RG >= FPLC_ROWS, "eight row groups cover the box");
 
+// inclusive min / max scan within each row of 16 lanes (lane 16r + 15
+// holds the row's result)
+__device__ __forceinline__ float row16_scan_min(float v) {
+  v = fminf(v, fpl_dpp<0x111>(v));
+  v = fminf(v, fpl_dpp<0x112>(v));
+  v = fminf(v, fpl_dpp<0x114>(v));
+  v = fminf(v, fpl_dpp<0x118>(v));
+  return v;
+}
+__device__ __forceinline__ float row16_scan_max(float v) {
+  v = fmaxf(v, fpl_dpp<0x111>(v));
+  v = fmaxf(v, fpl_dpp<0x112>(v));
+  v = fmaxf(v, fpl_dpp<0x114>(v));
+  v = fmaxf(v, fpl_dpp<0x118>(v));
+  return v;
+}
 // the value of the next lane (lane 63 keeps its own)
 __device__ __forceinline__ float lane_next(float v) { return fpl_dpp<0x130>(v); }
 
@@ -58,7 +76,7 @@ __device__ __forceinline__ float4 cell_at(unsigned lbase, unsigned i) {
 }
 
 __device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __restrict__ box,
-                                          float (*red)[4]) {
+                                          float (*red)[4], int4* __restrict__ btab) {
 #pragma clang fp contract(off)
   const int N = tk.N, T = tk.T;
   const float m = tk.mid;
@@ -102,36 +120,81 @@ __device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __r
   const int scol = (wave & 1) * C_COLS + lane;
   const int srow0 = (wave >> 1) * C_RG;
 
-  // the workgroup's box of the samples of steps [k0, min(XE, k0 + FPL_U - 1)]
-  // (computed by every wave, see fpl_box), and whether it fits the LDS
-  struct Box { int x0, y0, w, h; bool any, fits; };
-  const FplBoxGeom bg = fpl_box_geom(tk, ab * C_WAVES, C_WAVES, rb * 64, min(N - 1, rb * 64 + 63), lane);
-  auto box_of = [&](int k0) {
-    const FplBox b = fpl_box(bg, m, k0, min(XE, k0 + FPL_U - 1));
-    Box bx{b.x0, b.y0, b.w, b.h, b.any, false};
-    bx.fits = b.any && b.w > 0 && b.h > 0 && b.w <= FPLC_PITCH && b.h <= FPLC_ROWS;
+  // Boxes of the chunks, C_TAB at a time, in a table in LDS: thread t takes
+  // chunk kbase + t / 16 and angle t % 16 of the block, the four corners of
+  // its samples (first / last ray x first / last step), and a DPP scan over
+  // each 16-lane row gives the chunk's min / max at the row's last lane,
+  // which writes the entry -- the box that reducing over every (ray, angle,
+  // step) gives, because positions are monotone in the ray offset and the
+  // step (see fpl_box).  1-based pixel origin with two pixels of margin,
+  // size in cells; w = 0 when the chunk holds no active (ray, angle).
+  struct Box { int x0, y0, w, h; bool fits; };
+  const int ta = ab * C_WAVES + (threadIdx.x & 15);
+  const bool tact = ta < T;
+  const float2 tsc = const_f2(tk.sc, min(ta, T - 1));
+  const int r0 = rb * 64, r1 = min(N - 1, rb * 64 + 63);
+  const float tym0 = (float)(r0 + 1) - m, tym1 = (float)(r1 + 1) - m;
+  const float tpx0 = m - tsc.x * tym0, tpx1 = m - tsc.x * tym1;  // m - fl(s*ym)
+  const float tpy0 = m + tsc.y * tym0, tpy1 = m + tsc.y * tym1;  // m + fl(c*ym)
+  const int nch = XE >= XS ? (XE - XS) / FPL_U + 1 : 0;
+  auto fill_boxes = [&](int kbase) {
+    const int kc = kbase + (int)(threadIdx.x >> 4);
+    const int k0 = XS + kc * FPL_U, k1 = min(XE, k0 + FPL_U - 1);
+    float mnx = 1e30f, mny = 1e30f, mxx = -1e30f, mxy = -1e30f;
+    if (tact && kc < nch) {
+      const float ua = (float)k0 - m, ub = (float)k1 - m;
+      const float x00 = fmaf(tsc.y, ua, tpx0), x01 = fmaf(tsc.y, ub, tpx0);
+      const float x10 = fmaf(tsc.y, ua, tpx1), x11 = fmaf(tsc.y, ub, tpx1);
+      const float y00 = fmaf(tsc.x, ua, tpy0), y01 = fmaf(tsc.x, ub, tpy0);
+      const float y10 = fmaf(tsc.x, ua, tpy1), y11 = fmaf(tsc.x, ub, tpy1);
+      mnx = fminf(fminf(x00, x01), fminf(x10, x11)); mxx = fmaxf(fmaxf(x00, x01), fmaxf(x10, x11));
+      mny = fminf(fminf(y00, y01), fminf(y10, y11)); mxy = fmaxf(fmaxf(y00, y01), fmaxf(y10, y11));
+    }
+    mnx = row16_scan_min(mnx); mny = row16_scan_min(mny);
+    mxx = row16_scan_max(mxx); mxy = row16_scan_max(mxy);
+    if ((threadIdx.x & 15) == 15) {
+      int4 e = make_int4(0, 0, 0, 0);
+      if (mnx <= mxx) {
+        e.x = (int)floorf(fmaxf(mnx, -1e6f)) - 2;
+        e.y = (int)floorf(fmaxf(mny, -1e6f)) - 2;
+        e.z = (int)floorf(fminf(mxx, 1e6f)) - e.x + 4;
+        e.w = (int)floorf(fminf(mxy, 1e6f)) - e.y + 4;
+      }
+      btab[threadIdx.x >> 4] = e;
+    }
+  };
+  int kbase = 0;
+  auto box_at = [&](int kc) {
+    const int4 e = btab[kc - kbase];
+    Box bx{e.x, e.y, e.z, e.w, false};
+    bx.fits = bx.w > 0 && bx.h > 0 && bx.w <= FPLC_PITCH && bx.h <= FPLC_ROWS;
     return bx;
   };
   // This thread's pixels of a box: column scol (and its right neighbour
-  // through the next lane), rows srow0 .. srow0 + C_RG; zero outside the image.
+  // through the next lane), rows srow0 .. srow0 + C_RG.  Rows and columns
+  // are clamped into the image and loaded without conditions: a pixel
+  // outside the image only enters cells that no sample reads (the bounds
+  // test keeps taps in [2, N]; interior boxes lie inside the image).
   float v[C_RG + 1];
   auto load_pixels = [&](const Box& bx) {
-    const int gx = bx.x0 + scol;
-    const bool xin = bx.fits && scol <= bx.w && gx >= 1 && gx <= N;
+    if (!bx.fits) return;  // uniform
+    const int gx = min(max(bx.x0 + scol, 1), N) - 1;
 #pragma unroll
     for (int k = 0; k <= C_RG; ++k) {
-      const int gy = bx.y0 + srow0 + k;
-      v[k] = 0.0f;
-      if (xin && srow0 + k <= bx.h && gy >= 1 && gy <= N) v[k] = img[(size_t)(gy - 1) * N + (gx - 1)];
+      const int gy = min(max(bx.y0 + srow0 + k, 1), N) - 1;  // wave-uniform
+      v[k] = img[(size_t)gy * N + gx];
     }
   };
 
   // Software pipeline: the pixels of chunk k+1 are loaded into registers
   // while chunk k is sampled from LDS, and written as cells after the
   // barrier that ends chunk k.
-  Box cur = box_of(XS);
+  fill_boxes(0);
+  __syncthreads();
+  Box cur = nch > 0 ? box_at(0) : Box{0, 0, 0, 0, false};
   load_pixels(cur);
-  for (int c0 = XS; c0 <= XE; c0 += FPL_U) {
+  for (int kc = 0; kc < nch; ++kc) {
+    const int c0 = XS + kc * FPL_U;
     const int c1 = min(XE, c0 + FPL_U - 1);
     // (1) this chunk's cells from the registers (every wave passed the
     // barrier that ended the previous chunk)
@@ -145,9 +208,13 @@ __device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __r
         if (cw && srow0 + k < cur.h)
           box[(srow0 + k) * FPLC_PITCH + scol] = make_float4(v[k], nb[k] - v[k], v[k + 1], nb[k + 1] - v[k + 1]);
     }
+    if (kc + 1 == kbase + C_TAB && kc + 1 < nch) {  // uniform: the next table of boxes
+      kbase += C_TAB;
+      fill_boxes(kbase);
+    }
     __syncthreads();
     // (2) the next chunk's box, and its pixels in flight during (3)
-    const Box nxt = box_of(c0 + FPL_U);
+    const Box nxt = kc + 1 < nch ? box_at(kc + 1) : Box{0, 0, 0, 0, false};
     load_pixels(nxt);
     // (3) sample this chunk
     const int bx0 = cur.x0, by0 = cur.y0, bw = cur.w, bh = cur.h;
@@ -190,7 +257,7 @@ __device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __r
           }
         }
       }
-    } else if (cur.any && act) {
+    } else if (act) {
       // box too large for LDS: bilinear cells from global memory (as fpl.hip)
       const int P = N + 1;
       const float fN = (float)N;
@@ -216,13 +283,14 @@ __device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __r
 __global__ __launch_bounds__(C_THREADS) void fplc_kernel(const FPLTask* __restrict__ tasks, int ntasks, int base,
                                                           int nblocks) {
   // dynamic LDS: the cell box at address 0 (so a cell's address is one
-  // multiply-add), then per-wave min x, min y, max x, max y (or step range)
+  // multiply-add), then per-wave min / max scratch, then the table of boxes
   extern __shared__ float4 cbox[];
   float (*red)[4] = reinterpret_cast<float (*)[4]>(cbox + FPLC_CAP);
+  int4* btab = reinterpret_cast<int4*>(cbox + FPLC_CAP + C_WAVES);
   const int lbk = xcd_remap(blockIdx.x, gridDim.x);
   if (lbk >= nblocks) return;
   const int gb = base + lbk;
-  fplc_body(tasks[fpl_find_task(tasks, ntasks, gb)], gb, cbox, red);
+  fplc_body(tasks[fpl_find_task(tasks, ntasks, gb)], gb, cbox, red, btab);
 }
 
 }  // namespace
