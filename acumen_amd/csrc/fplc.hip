@@ -27,8 +27,10 @@
 //
 // Interior chunks (every cell of the box passes the bounds test) use
 // positions by FMA from the ray's start; border chunks use the reference's
-// FP32 positions and bounds test; chunks whose box exceeds the LDS read the
-// global cells of fp.hip, exactly as fpl.hip does.
+// FP32 positions and bounds test.  A chunk whose box exceeds the cell box
+// stages raw pixels into the same LDS instead (up to C_RP x C_RROWS, read
+// as fpl.hip reads them, same arithmetic); only a box beyond that reads the
+// global cells of fp.hip.
 #include "fpl_common.hpp"
 
 #include <algorithm>
@@ -42,6 +44,10 @@ constexpr int C_THREADS = 64 * C_WAVES;
 constexpr int C_COLS = 63;                 // cells per column group
 constexpr int C_RG = 13;                   // cell rows per row group
 constexpr int C_TAB = C_THREADS / 16;     // chunk boxes per table (16 threads per chunk)
+constexpr int C_RP = 201;                  // raw-pixel box: row pitch (odd; C_RP + 1 <= 255 dwords)
+constexpr int C_RROWS = FPLC_CAP * 4 / C_RP;  // rows of a raw box in the cell box's bytes
+static_assert(C_RP + 1 <= 255, "ds_read2 offsets");
+static_assert(C_RROWS * C_RP <= FPLC_CAP * 4, "raw box within the cell box's LDS");
 constexpr int C_LDS = FPLC_CAP * 16 + C_WAVES * 16 + C_TAB * 16;  // dynamic LDS: box, scratch, table
 static_assert(C_LDS <= 163840, "LDS of one CU");
 static_assert(2 * C_COLS >= FPLC_PITCH, "two column groups cover a box row");
@@ -128,7 +134,7 @@ __device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __r
   // step) gives, because positions are monotone in the ray offset and the
   // step (see fpl_box).  1-based pixel origin with two pixels of margin,
   // size in cells; w = 0 when the chunk holds no active (ray, angle).
-  struct Box { int x0, y0, w, h; bool fits; };
+  struct Box { int x0, y0, w, h; bool fits, raw; };  // fits: cell box; raw: raw-pixel box
   const int ta = ab * C_WAVES + (threadIdx.x & 15);
   const bool tact = ta < T;
   const float2 tsc = const_f2(tk.sc, min(ta, T - 1));
@@ -166,8 +172,9 @@ __device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __r
   int kbase = 0;
   auto box_at = [&](int kc) {
     const int4 e = btab[kc - kbase];
-    Box bx{e.x, e.y, e.z, e.w, false};
+    Box bx{e.x, e.y, e.z, e.w, false, false};
     bx.fits = bx.w > 0 && bx.h > 0 && bx.w <= FPLC_PITCH && bx.h <= FPLC_ROWS;
+    bx.raw = !bx.fits && bx.w > 0 && bx.h > 0 && bx.w <= C_RP && bx.h <= C_RROWS;
     return bx;
   };
   // This thread's pixels of a box: column scol (and its right neighbour
@@ -191,7 +198,7 @@ __device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __r
   // barrier that ends chunk k.
   fill_boxes(0);
   __syncthreads();
-  Box cur = nch > 0 ? box_at(0) : Box{0, 0, 0, 0, false};
+  Box cur = nch > 0 ? box_at(0) : Box{0, 0, 0, 0, false, false};
   load_pixels(cur);
   for (int kc = 0; kc < nch; ++kc) {
     const int c0 = XS + kc * FPL_U;
@@ -207,6 +214,25 @@ __device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __r
       for (int k = 0; k < C_RG; ++k)
         if (cw && srow0 + k < cur.h)
           box[(srow0 + k) * FPLC_PITCH + scol] = make_float4(v[k], nb[k] - v[k], v[k + 1], nb[k + 1] - v[k + 1]);
+    } else if (cur.raw) {
+      // a box too large for the cells: raw pixels, row pitch C_RP, staged
+      // here without the prefetch (4 column groups of 64 x rows by wave
+      // quarter, 8 loads in flight; clamped loads as above)
+      float* rbox = reinterpret_cast<float*>(box);
+      const int col = (wave & 3) * 64 + lane;
+      const int gx = min(max(cur.x0 + col, 1), N) - 1;
+      const bool cok = col < cur.w;
+      for (int r0w = wave >> 2; r0w < cur.h; r0w += 4 * 8) {
+        float t[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int gy = min(max(cur.y0 + r0w + 4 * k, 1), N) - 1;  // wave-uniform
+          t[k] = img[(size_t)gy * N + gx];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (cok && r0w + 4 * k < cur.h) rbox[(r0w + 4 * k) * C_RP + col] = t[k];
+      }
     }
     if (kc + 1 == kbase + C_TAB && kc + 1 < nch) {  // uniform: the next table of boxes
       kbase += C_TAB;
@@ -214,7 +240,7 @@ __device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __r
     }
     __syncthreads();
     // (2) the next chunk's box, and its pixels in flight during (3)
-    const Box nxt = kc + 1 < nch ? box_at(kc + 1) : Box{0, 0, 0, 0, false};
+    const Box nxt = kc + 1 < nch ? box_at(kc + 1) : Box{0, 0, 0, 0, false, false};
     load_pixels(nxt);
     // (3) sample this chunk
     const int bx0 = cur.x0, by0 = cur.y0, bw = cur.w, bh = cur.h;
@@ -253,6 +279,44 @@ __device__ __forceinline__ void fplc_body(const FPLTask& tk, int gb, float4* __r
             const float4 q = box[__mul24(y1 - by0, FPLC_PITCH) + (x1 - bx0)];
             const float top = fmaf(fx, q.y, q.x);
             const float bot = fmaf(fx, q.w, q.z);
+            acc += fmaf(fy, bot - top, top);
+          }
+        }
+      }
+    } else if (cur.raw) {
+      const float* rbox = reinterpret_cast<const float*>(box);
+      const bool interior = bx0 >= 2 && by0 >= 2 && bx0 + bw - 1 <= N && by0 + bh - 1 <= N;
+      if (interior) {
+        // fpl.hip's fast path on raw pixels (same arithmetic as the cells)
+        const float cj = act ? c : 0.0f, sj = act ? s : 0.0f;
+        const float bxl = act ? (m - sy) - (float)bx0 : 1.0f;
+        const float byl = act ? (m + cyv) - (float)by0 : 1.0f;
+        float u = (float)c0 - m;
+#pragma unroll 4
+        for (int x = c0; x <= c1; ++x, u += 1.0f) {
+          const float xl = fmaf(cj, u, bxl);
+          const float yl = fmaf(sj, u, byl);
+          const int ix = (int)xl, iy = (int)yl;
+          const float fx = __builtin_amdgcn_fractf(xl), fy = __builtin_amdgcn_fractf(yl);
+          const float* p0 = rbox + (__mul24(iy, C_RP) + ix);
+          const float v00 = p0[0], v10 = p0[1], v01 = p0[C_RP], v11 = p0[C_RP + 1];
+          const float top = fmaf(fx, v10 - v00, v00);
+          const float bot = fmaf(fx, v11 - v01, v01);
+          acc += fmaf(fy, bot - top, top);
+        }
+      } else if (act) {
+        for (int x = c0; x <= c1; ++x) {
+          const float u = (float)x - m;
+          const float xr = (c * u - sy) + m;
+          const float yr = (s * u + cyv) + m;
+          const float fxr = floorf(xr), fyr = floorf(yr);
+          const int x1 = (int)fxr, y1 = (int)fyr;
+          if (x1 >= 2 && x1 <= N - 1 && y1 >= 2 && y1 <= N - 1) {
+            const float fx = xr - fxr, fy = yr - fyr;
+            const float* p0 = rbox + (__mul24(y1 - by0, C_RP) + (x1 - bx0));
+            const float v00 = p0[0], v10 = p0[1], v01 = p0[C_RP], v11 = p0[C_RP + 1];
+            const float top = fmaf(fx, v10 - v00, v00);
+            const float bot = fmaf(fx, v11 - v01, v01);
             acc += fmaf(fy, bot - top, top);
           }
         }
@@ -296,7 +360,8 @@ __global__ __launch_bounds__(C_THREADS) void fplc_kernel(const FPLTask* __restri
 }  // namespace
 
 bool fplc_may_overflow(int N, int T, const float* theta_deg) {
-  // as fpl_may_overflow for 16-angle blocks, against the cell box
+  // as fpl_may_overflow for 16-angle blocks; a chunk reads the global cells
+  // only when its box exceeds the raw-pixel box as well
   const double pi = 3.141592653589793;
   const double r = 0.70711 * ((double)N + 2.0);
   const double foot = std::hypot(63.0, (double)(FPL_U - 1));
@@ -311,7 +376,7 @@ bool fplc_may_overflow(int N, int T, const float* theta_deg) {
     span = std::max(span, hi - lo);
   }
   const double extent = foot + (span + 1e-6) * r + 8.0;
-  return extent > (double)std::min(FPLC_PITCH, FPLC_ROWS);
+  return extent > (double)std::min(C_RP, C_RROWS);  // beyond the raw-pixel box: global cells
 }
 
 int fplc_blocks(int N, int T) { return ((N + 63) / 64) * ((T + C_WAVES - 1) / C_WAVES); }
