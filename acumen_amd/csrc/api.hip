@@ -761,9 +761,12 @@ uint64_t theta_hash(const std::vector<float>& th) {
   return h;
 }
 
-// Average ms of one radon of slice sl (FBP image -> ruk, contents
-// irrelevant to the timing) with variant v: cells when v reads them, then
-// the projector, launched back to back between two events.
+// Average ms per slice of one radon of slice sl (FBP image -> ruk, contents
+// irrelevant to the timing) with variant v, as a batched launch would run
+// it: the cells pass (when v reads it) once, plus the projector launched
+// over R copies of the slice divided by R, R chosen so the launch has at
+// least two workgroups per slot of the GPU (a single small slice would
+// leave CUs idle that a batch fills).  The copies write scratch outputs.
 double time_variant(acu_batch* b, int sl, int v, int reps) {
   Slice& s = b->s[sl];
   acu_ctx* c = b->ctx;
@@ -776,12 +779,21 @@ double time_variant(acu_batch* b, int sl, int v, int reps) {
   CellTask ct = cell_task(s, s.fbp);
   ct.block_begin = 0;
   cl.upload({ct});
-  int nfl = 0, nfp = 0, nfs = 0;
+  int nfl = 0, nfp = 0, nfs = 0, R = 1;
+  struct Scratch { float* p = nullptr; ~Scratch() { if (p) hipFree(p); } } sc;
   if (v > 0) {
-    FPLTask t = fpl_task(s, s.fbp, s.ruk);
-    t.block_begin = 0;
-    fl.upload({t});
-    nfl = fpl_blocks(s.N, s.T, v);
+    const int per = fpl_blocks(s.N, s.T, v);
+    const int slots = v == FPV_CELL ? 256 : 1024;  // workgroups resident on the GPU at once
+    R = std::max(1, std::min(8, (2 * slots + per - 1) / per));
+    if (R > 1) ACU_HIP_CHECK(hipMalloc(&sc.p, (size_t)(R - 1) * s.N * s.T * sizeof(float)));
+    std::vector<FPLTask> tv;
+    for (int r = 0; r < R; ++r) {
+      FPLTask t = fpl_task(s, s.fbp, r == 0 ? s.ruk : sc.p + (size_t)(r - 1) * s.N * s.T);
+      t.block_begin = r * per;
+      tv.push_back(t);
+    }
+    fl.upload(tv);
+    nfl = R * per;
   } else {
     FPTask t = fp_task(s, s.ruk);
     t.block_begin = 0;
@@ -790,25 +802,27 @@ double time_variant(acu_batch* b, int sl, int v, int reps) {
     nfp = fp_ray_blocks(s.N) * s.T * fp_chunks(s.N, s.fp_clen);
     nfs = fp_sum_blocks(s.N, s.T);
   }
-  auto run = [&] {
-    if (cells) launch_cells(cl.d, 1, 0, cell_blocks(s.N), c->stream);
-    if (v > 0) launch_forward_lds(fl.d, 1, 0, nfl, v, c->stream);
-    else launch_forward(fp.d, 1, 0, nfp, 0, nfs, c->stream);
+  auto timed_ms = [&](auto&& launch) {
+    hipEvent_t e0 = c->ev(), e1 = c->ev();
+    launch();  // warm-up
+    ACU_HIP_CHECK(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) launch();
+    ACU_HIP_CHECK(hipEventRecord(e1, c->stream));
+    check_launch();
+    ACU_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    ACU_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    c->pool.push_back(e0);
+    c->pool.push_back(e1);
+    return (double)ms / reps;
   };
-  hipEvent_t e0 = c->ev(), e1 = c->ev();
-  run();  // warm-up
-  ACU_HIP_CHECK(hipEventRecord(e0, c->stream));
-  for (int r = 0; r < reps; ++r) run();
-  ACU_HIP_CHECK(hipEventRecord(e1, c->stream));
-  check_launch();
-  ACU_HIP_CHECK(hipEventSynchronize(e1));
-  float ms = 0.f;
-  ACU_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-  c->pool.push_back(e0);
-  c->pool.push_back(e1);
+  double ms = 0.0;
+  if (cells) ms += timed_ms([&] { launch_cells(cl.d, 1, 0, cell_blocks(s.N), c->stream); });
+  if (v > 0) ms += timed_ms([&] { launch_forward_lds(fl.d, R, 0, nfl, v, c->stream); }) / R;
+  else ms += timed_ms([&] { launch_forward(fp.d, 1, 0, nfp, 0, nfs, c->stream); });
   cl.release(); fl.release(); fp.release();
   s.fpv = saved;
-  return (double)ms / reps;
+  return ms;
 }
 
 const char* variant_name(int v) {
