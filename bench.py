@@ -493,7 +493,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 2; ingest: 50)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=4, help="slices per step")
+    ap.add_argument("--batch", type=int, default=16, help="slices per step (production workload)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-slice leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
