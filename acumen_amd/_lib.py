@@ -61,6 +61,8 @@ SIGNATURES = {
     "acu_batch_set_ramp_variant": (_c_int, [_vp, _c_int]),
     "acu_batch_ramp_variant": (_c_int, [_vp]),
     "acu_batch_set_fp_variant": (_c_int, [_vp, _c_int, _c_int]),
+    "acu_batch_set_bp_tile": (_c_int, [_vp, _c_int]),
+    "acu_batch_bp_tile": (_c_int, [_vp]),
     "acu_radon_adjoint_dev": (_c_int, [_vp, _c_int, _vp, _vp]),
     "acu_radon_adjoint": (_c_int, [_vp, _fp, _fp]),
     "acu_batch_loss_grad": (_c_int, [_vp, ctypes.POINTER(_fp), ctypes.POINTER(_fp), _c_float, _c_float,

@@ -56,12 +56,14 @@ __device__ __forceinline__ int xcd_remap(int b, int grid) {
 }
 
 // ---- backprojection tiling (bp.hip) ----
+// A tile is 64 (x) x ty (y) pixels, ty = 16, 32 or 64 (a per-batch choice),
+// one workgroup of 4 waves (8 at ty = 64).  Tile activity is kept per
+// 64 x 16 sub-tile, so every tile height reads the same table.
 constexpr int BP_TX = 64;        // pixels along x per tile (one per lane)
-constexpr int BP_TY = 32;        // pixels along y per tile
-constexpr int BP_THREADS = 256;  // 4 waves; each wave owns BP_TY/4 rows
-constexpr int BP_PY = BP_TY / (BP_THREADS / BP_TX);  // rows per thread
+constexpr int BP_SUB = 16;       // rows of one activity sub-tile
+constexpr int BP_TY_DEFAULT = 32;
 constexpr int BP_AB = 32;        // angles staged per LDS block
-constexpr int BP_W = 76;         // detector bins staged per angle (>= ceil(hypot(63,31)) + 4)
+inline bool bp_tile_ok(int ty) { return ty == 16 || ty == 32 || ty == 64; }
 
 // ---- forward projection (fp.hip) ----
 constexpr int FP_THREADS = 256;  // rays per workgroup (one angle per workgroup)
@@ -112,11 +114,12 @@ struct BPTask {
   int N, T;
   float mid;             // (N+1)/2 with integer division
   int tile_begin;        // first global tile index of this slice
-  int ntx, nty;          // tiles along x and y
+  int ntx, nty;          // tiles along x and y (at the launch's tile height)
+  int nsub;              // 16-row sub-tiles along y, ceil(N / 16)
   const float* sino;     // filtered sinogram (N*T)
   const float* cmask;    // N*N
   const float2* sc;      // per-angle (sin, cos) for backprojection
-  const uint8_t* tile_active;  // nty*ntx: 1 if any cmask > 0 in the tile
+  const uint8_t* tile_active;  // nsub*ntx: 1 if any cmask > 0 in the 64 x 16 sub-tile
   float* img;            // N*N out
 };
 
@@ -188,7 +191,7 @@ int fft_len(int N);
 bool fft_supported(int N);
 void fft_tables(int N, const float* taps, float* G, float2* tw);
 void launch_ramp_fft(const FFTTask* d_tasks, int ntasks, int base, int blocks, int maxL, hipStream_t s);
-void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, hipStream_t s);
+void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, int ty, hipStream_t s);
 void launch_cells(const CellTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
 void launch_forward(const FPTask* d_tasks, int ntasks, int base, int blocks, int sum_base,
                     int sum_blocks, hipStream_t s);
@@ -199,7 +202,7 @@ void launch_forward_cell(const FPLTask* d_tasks, int ntasks, int base, int block
 
 // Host-side helpers.
 int bp_tiles_x(int N);
-int bp_tiles_y(int N);
+int bp_tiles_y(int N, int ty);
 int rf_tiles_m(int N);
 int rf_tiles_n(int T);
 int fp_ray_blocks(int N);

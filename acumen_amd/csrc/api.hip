@@ -88,8 +88,8 @@ void make_cmask(int N, int T, int wpu, float* c) {
   const float pi = ref_pi();
   const float rad_n = (float)(N - 1) / 2.0f - (float)wpu;
   const float half = ((float)N - 1.0f) / 2.0f;
-  for (int jx = 1; jx <= N; ++jx)
-    for (int jy = 1; jy <= N; ++jy) {
+  for (int jy = 1; jy <= N; ++jy)  // each pixel on its own: rows in memory order
+    for (int jx = 1; jx <= N; ++jx) {
       const float dx = ((float)jx - half) - 1.0f;
       const float dy = ((float)jy - half) - 1.0f;
       const float rr = sqrtf(dx * dx + dy * dy);
@@ -149,7 +149,7 @@ struct Slice {
   float mid = 0.f;
   std::vector<float> theta;
   bool theta_set = false;
-  int ntx = 0, nty = 0;
+  int ntx = 0, nty = 0, nsub = 0;  // backprojection tiles along x, y; 16-row sub-tiles along y
   // operator arena
   float *h = nullptr, *cmask = nullptr;
   float2 *scbp = nullptr, *scfp = nullptr;
@@ -224,6 +224,7 @@ struct acu_batch {
       sw_total = 0;
   int fl_total[4] = {0, 0, 0, 0};  // LDS projector blocks per variant group (lds1, lds2, lds4, cell16)
   int ramp_variant = acu::RAMP_FP64;  // arithmetic of the ramp filter
+  int bp_ty = acu::BP_TY_DEFAULT;     // backprojection tile height (16, 32 or 64)
   // task sets for the fixed operand combinations of FBP / TV
   acu::RFSet rf_f, rf_fk, rf_ruk, rf_rp;  // ramp filter of f, fk, ruk, rp
   acu::DevArray<acu::BPTask> bp_fbp, bp_ifkt, bp_iukt, bp_irpt;
@@ -282,7 +283,7 @@ void layout_ops(acu_batch* b, Carver& cv) {
     s.cmask = cv.take<float>(NN);
     s.scbp = cv.take<float2>(s.T);
     s.scfp = cv.take<float2>(s.T);
-    s.tact = cv.take<uint8_t>((size_t)s.ntx * s.nty);
+    s.tact = cv.take<uint8_t>((size_t)s.ntx * s.nsub);
     s.f = cv.take<float>(NT);
     s.sinoCS = cv.take<float>(NT);
     s.fbp = cv.take<float>(NN);
@@ -318,10 +319,10 @@ void upload_tables(acu_batch* b, int i) {
   std::vector<float> h(2 * s.N + 1), cm((size_t)s.N * s.N);
   make_filter(s.N, h.data());
   make_cmask(s.N, s.T, s.wpu, cm.data());
-  std::vector<uint8_t> act((size_t)s.ntx * s.nty, 0);
+  std::vector<uint8_t> act((size_t)s.ntx * s.nsub, 0);
   for (int y = 0; y < s.N; ++y)
     for (int x = 0; x < s.N; ++x)
-      if (cm[(size_t)y * s.N + x] != 0.0f) act[(size_t)(y / BP_TY) * s.ntx + x / BP_TX] = 1;
+      if (cm[(size_t)y * s.N + x] != 0.0f) act[(size_t)(y / BP_SUB) * s.ntx + x / BP_TX] = 1;
   ACU_HIP_CHECK(hipMemcpy(s.h, h.data(), h.size() * 4, hipMemcpyHostToDevice));
   s.ff_edge = h[2 * s.N - 1];
   if (s.fftG) {
@@ -334,6 +335,22 @@ void upload_tables(acu_batch* b, int i) {
   }
   ACU_HIP_CHECK(hipMemcpy(s.cmask, cm.data(), cm.size() * 4, hipMemcpyHostToDevice));
   ACU_HIP_CHECK(hipMemcpy(s.tact, act.data(), act.size(), hipMemcpyHostToDevice));
+}
+
+// Tables of slice i from slice j of the same (N, T, wpu), device to device:
+// a batch builds its host tables once per distinct geometry.
+void copy_tables(acu_batch* b, int j, int i) {
+  const auto& a = b->s[j];
+  auto& s = b->s[i];
+  ACU_HIP_CHECK(hipMemcpy(s.h, a.h, (size_t)(2 * s.N + 1) * 4, hipMemcpyDeviceToDevice));
+  ACU_HIP_CHECK(hipMemcpy(s.cmask, a.cmask, (size_t)s.N * s.N * 4, hipMemcpyDeviceToDevice));
+  ACU_HIP_CHECK(hipMemcpy(s.tact, a.tact, (size_t)s.ntx * s.nsub, hipMemcpyDeviceToDevice));
+  s.ff_edge = a.ff_edge;
+  if (s.fftG) {
+    const size_t L = (size_t)fft_len(s.N);
+    ACU_HIP_CHECK(hipMemcpy(s.fftG, a.fftG, L * 4, hipMemcpyDeviceToDevice));
+    ACU_HIP_CHECK(hipMemcpy(s.fftW, a.fftW, L / 2 * sizeof(float2), hipMemcpyDeviceToDevice));
+  }
 }
 
 uint64_t theta_hash(const std::vector<float>& th);
@@ -379,7 +396,7 @@ FFTTask ff_task(const Slice& s, const float* in, float* out) {
   return FFTTask{s.N, s.T, L, lg, s.ff_begin, L == 2 * s.N - 2 ? 1 : 0, s.ff_edge, s.fftG, s.fftW, in, out};
 }
 BPTask bp_task(const Slice& s, const float* in, float* out) {
-  return BPTask{s.N, s.T, s.mid, s.bp_begin, s.ntx, s.nty, in, s.cmask, s.scbp, s.tact, out};
+  return BPTask{s.N, s.T, s.mid, s.bp_begin, s.ntx, s.nty, s.nsub, in, s.cmask, s.scbp, s.tact, out};
 }
 FPLTask fpl_task(const Slice& s, const float* img, float* out) {
   static const bool fixed_pitch = [] {
@@ -464,6 +481,28 @@ void refresh_fp(acu_batch* b) {
   if (b->graph) { ACU_HIP_CHECK(hipGraphExecDestroy(b->graph)); b->graph = nullptr; }
 }
 
+// Backprojection tile ranges at the batch's tile height.
+void assign_bp(acu_batch* b) {
+  int bp = 0;
+  for (auto& s : b->s) {
+    s.nty = bp_tiles_y(s.N, b->bp_ty);
+    s.bp_begin = bp; s.bp_n = s.ntx * s.nty; bp += s.bp_n;
+  }
+  b->bp_total = bp;
+}
+
+void build_op_tasks(acu_batch* b);
+
+// A new backprojection tile height: new ranges and task arrays, and a
+// captured outer iteration is rebuilt on the next tvopt.
+void set_bp_tile(acu_batch* b, int ty) {
+  b->bp_ty = ty;
+  assign_bp(b);
+  build_op_tasks(b);
+  if (b->tv_mem) build_tv_tasks(b);
+  if (b->graph) { ACU_HIP_CHECK(hipGraphExecDestroy(b->graph)); b->graph = nullptr; }
+}
+
 void build_op_tasks(acu_batch* b) {
   build_rfset(b, b->rf_f, [](Slice& s) { return (const float*)s.f; }, [](Slice& s) { return s.sinoCS; });
   build(b, b->bp_fbp, [](Slice& s) { return bp_task(s, s.sinoCS, s.fbp); });
@@ -537,7 +576,7 @@ void iradon_seq(acu_batch* b, const RFSet& rf, const DevArray<BPTask>& bp, int s
   acu_ctx* c = b->ctx;
   const Range r2 = bp_range(b, sl);
   ramp_seq(b, rf, sl);
-  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, c->stream); });
+  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, b->bp_ty, c->stream); });
   check_launch();
 }
 
@@ -749,7 +788,7 @@ void bp_dev(acu_batch* b, int sl, float* img, const float* filt) {
   BPTask t = bp_task(s, filt, img);
   t.tile_begin = 0;
   BPTask* d = put_adhoc(b, t);
-  timed(c, K_BP, [&] { launch_backproject(d, 1, 0, s.bp_n, c->stream); });
+  timed(c, K_BP, [&] { launch_backproject(d, 1, 0, s.bp_n, b->bp_ty, c->stream); });
   check_launch();
 }
 
@@ -858,7 +897,63 @@ double time_ramp(acu_batch* b, int v, int reps) {
   return (double)ms / reps;
 }
 
-std::string autotune(acu_batch* b, int reps) {
+// Slices the backprojection timing covers: the batch's first slices up to
+// 8192 tiles of 64 x 32 (about five rounds of the GPU's workgroup slots),
+// at least one -- enough to time a full GPU without launching the whole of
+// a large batch three times.
+int bp_timing_slices(const acu_batch* b) {
+  int n = 0, tiles = 0;
+  while (n < (int)b->s.size() && (n == 0 || tiles < 8192)) {
+    tiles += b->s[n].ntx * bp_tiles_y(b->s[n].N, 32);
+    ++n;
+  }
+  return n;
+}
+
+// Average ms of one batched backprojection of those slices (sinoCS ->
+// scratch images, contents irrelevant to the timing) at tile height ty.
+double time_bp(acu_batch* b, int ty, int reps) {
+  acu_ctx* c = b->ctx;
+  const int ns = bp_timing_slices(b);
+  size_t total = 0;
+  for (int i = 0; i < ns; ++i) total += (size_t)b->s[i].N * b->s[i].N;
+  struct Scratch { float* p = nullptr; ~Scratch() { if (p) hipFree(p); } } sc;
+  ACU_HIP_CHECK(hipMalloc(&sc.p, total * sizeof(float)));
+  std::vector<BPTask> tv;
+  int tiles = 0;
+  size_t off = 0;
+  for (int i = 0; i < ns; ++i) {
+    const Slice& s = b->s[i];
+    BPTask t = bp_task(s, s.sinoCS, sc.p + off);
+    t.nty = bp_tiles_y(s.N, ty);
+    t.tile_begin = tiles;
+    tiles += s.ntx * t.nty;
+    off += (size_t)s.N * s.N;
+    tv.push_back(t);
+  }
+  DevArray<BPTask> arr;
+  arr.upload(tv);
+  auto run = [&] { launch_backproject(arr.d, arr.n, 0, tiles, ty, c->stream); };
+  hipEvent_t e0 = c->ev(), e1 = c->ev();
+  run();
+  ACU_HIP_CHECK(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r) run();
+  ACU_HIP_CHECK(hipEventRecord(e1, c->stream));
+  check_launch();
+  ACU_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  ACU_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  c->pool.push_back(e0);
+  c->pool.push_back(e1);
+  arr.release();
+  return (double)ms / reps;
+}
+
+// What autotune times: the projector variants (per slice shape), the ramp
+// arithmetic and the backprojection tile height (per batch).
+constexpr unsigned TUNE_FP = 1, TUNE_RAMP = 2, TUNE_BP = 4, TUNE_ALL = 7;
+
+std::string autotune(acu_batch* b, int reps, unsigned what = TUNE_ALL) {
   require_theta(b);
   b->ctx->sync();
   static const int cand[5] = {1, 2, 4, FPV_CELL, 0};
@@ -866,7 +961,7 @@ std::string autotune(acu_batch* b, int reps) {
   std::map<std::tuple<int, int, uint64_t>, std::pair<int, std::vector<double>>> seen;
   std::ostringstream js;
   js << "{\"reps\":" << reps << ",\"slices\":[";
-  for (size_t i = 0; i < b->s.size(); ++i) {
+  for (size_t i = 0; i < ((what & TUNE_FP) ? b->s.size() : 0); ++i) {
     Slice& s = b->s[i];
     const auto key = std::make_tuple(s.N, s.T, theta_hash(s.theta));
     auto it = seen.find(key);
@@ -888,7 +983,7 @@ std::string autotune(acu_batch* b, int reps) {
   js << "]";
   // ramp filter: the fastest arithmetic among those that meet the FBP / TV
   // accuracy bar (bf16x3, with ~16 bits per operand, does not)
-  {
+  if (what & TUNE_RAMP) {
     static const int rc[4] = {RAMP_FP64, RAMP_FP32, RAMP_BF16X6, RAMP_FFT};
     bool fft_ok = true;
     for (auto& s : b->s) fft_ok = fft_ok && s.fftG != nullptr;
@@ -905,6 +1000,20 @@ std::string autotune(acu_batch* b, int reps) {
     }
     b->ramp_variant = best;
     js << "},\"choice\":\"" << ramp_name(best) << "\"}";
+  }
+  // backprojection tile height: the fastest over the batch's first slices
+  if (what & TUNE_BP) {
+    static const int tys[3] = {16, 32, 64};
+    int best = b->bp_ty;
+    double best_ms = -1.0;
+    js << ",\"bp\":{\"ms\":{";
+    for (int k = 0; k < 3; ++k) {
+      const double ms = time_bp(b, tys[k], reps);
+      js << (k ? "," : "") << "\"" << tys[k] << "\":" << ms;
+      if (best_ms < 0.0 || ms < best_ms) { best = tys[k]; best_ms = ms; }
+    }
+    js << "},\"choice\":" << best << "}";
+    if (best != b->bp_ty) set_bp_tile(b, best);
   }
   js << "}";
   refresh_fp(b);
@@ -1091,7 +1200,7 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
   try {
     ACU_HIP_CHECK(hipSetDevice(ctx->device));
     b->s.resize(nslices);
-    int rf = 0, bp = 0, ew = 0, sw = 0;
+    int rf = 0, ew = 0, sw = 0;
     // forward-projection variant per slice: ACUMEN_FP_PATH = lds (default) |
     // gather; ACUMEN_FPL_AL = 1 | 2 | 4 forces the LDS angles per lane;
     // acu_batch_autotune replaces both by measurement
@@ -1105,6 +1214,11 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       b->ramp_variant = r == "fp32" ? acu::RAMP_FP32 : r == "bf16x3" ? acu::RAMP_BF16X3
                         : r == "bf16x6" ? acu::RAMP_BF16X6 : r == "fft" ? acu::RAMP_FFT : acu::RAMP_FP64;
     }
+    // backprojection tile height: ACUMEN_BP_TILE = 16 | 32 (default) | 64
+    if (const char* tv = std::getenv("ACUMEN_BP_TILE")) {
+      const int ty = std::atoi(tv);
+      if (acu::bp_tile_ok(ty)) b->bp_ty = ty;
+    }
     // tuning: forward-projection sweep chunk length (steps), ACUMEN_FP_CHUNK
     const char* ev = std::getenv("ACUMEN_FP_CHUNK");
     const int fp_clen_env = ev ? std::atoi(ev) : 0;
@@ -1112,12 +1226,11 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       auto& s = b->s[i];
       s.N = nimage[i]; s.T = ntheta[i]; s.wpu = wpu[i];
       s.mid = (float)((s.N + 1) / 2);
-      s.ntx = acu::bp_tiles_x(s.N); s.nty = acu::bp_tiles_y(s.N);
+      s.ntx = acu::bp_tiles_x(s.N); s.nsub = (s.N + acu::BP_SUB - 1) / acu::BP_SUB;
       s.theta.assign(s.T, 0.0f);
       s.rf_begin = rf; s.rf_n = acu::rf_tiles_m(s.N) * acu::rf_tiles_n(s.T); rf += s.rf_n;
       s.ff_begin = b->ff_total; s.ff_n = (s.T + 1) / 2; b->ff_total += s.ff_n;
       b->ff_maxL = std::max(b->ff_maxL, acu::fft_len(s.N));
-      s.bp_begin = bp; s.bp_n = s.ntx * s.nty; bp += s.bp_n;
       s.fp_clen = fp_clen_env > 0 ? fp_clen_env : acu::FP_CHUNK_DEFAULT;
       s.fpv = gather ? 0
                      : (al_env == 1 || al_env == 2 || al_env == 4 || al_env == acu::FPV_CELL) ? al_env
@@ -1125,15 +1238,24 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       s.ew_begin = ew; s.ew_n = acu::ew_blocks((size_t)s.N * s.N); ew += s.ew_n;
       s.sw_begin = sw; s.sw_n = acu::ew_blocks((size_t)s.N * s.T); sw += s.sw_n;
     }
-    b->rf_total = rf; b->bp_total = bp;
+    b->rf_total = rf;
+    acu::assign_bp(b.get());
     b->ew_total = ew; b->sw_total = sw;
     acu::Carver dry;
     acu::layout_ops(b.get(), dry);
     ACU_HIP_CHECK(hipMalloc(&b->ops_mem, dry.off + 256));
     acu::Carver cv{static_cast<char*>(b->ops_mem), 0};
     acu::layout_ops(b.get(), cv);
+    std::map<std::tuple<int, int, int>, int> built;  // (N, T, wpu) -> first slice with it
     for (int i = 0; i < nslices; ++i) {
-      acu::upload_tables(b.get(), i);
+      const auto key = std::make_tuple(nimage[i], ntheta[i], wpu[i]);
+      const auto it = built.find(key);
+      if (it == built.end()) {
+        acu::upload_tables(b.get(), i);
+        built.emplace(key, i);
+      } else {
+        acu::copy_tables(b.get(), it->second, i);
+      }
       ACU_HIP_CHECK(hipMemset(b->s[i].f, 0, (size_t)b->s[i].N * b->s[i].T * 4));
       if (theta_deg && theta_deg[i]) {
         std::memcpy(b->s[i].theta.data(), theta_deg[i], sizeof(float) * b->s[i].T);
@@ -1338,6 +1460,17 @@ int acu_batch_ramp_variant(const acu_batch* b) {
   return b->ramp_variant;
 }
 
+int acu_batch_set_bp_tile(acu_batch* b, int rows) {
+  if (!b) return fail(ACU_ERR_ARG, "null batch");
+  if (!acu::bp_tile_ok(rows)) return fail(ACU_ERR_ARG, "acu_batch_set_bp_tile: rows must be 16, 32 or 64");
+  ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync(); acu::set_bp_tile(b, rows));
+}
+
+int acu_batch_bp_tile(const acu_batch* b) {
+  if (!b) return -fail(ACU_ERR_ARG, "null batch");
+  return b->bp_ty;
+}
+
 int acu_batch_fp_variant(const acu_batch* b, int slice) {
   if (!b || slice < 0 || slice >= (int)b->s.size()) return -fail(ACU_ERR_ARG, "acu_batch_fp_variant: bad slice");
   return b->s[slice].fpv;
@@ -1493,6 +1626,7 @@ acu_batch* make_volume_batch(acu_ctx* c, int n, int N, int T, int wpu, const flo
     b->ramp_variant = like->ramp_variant;
     for (auto& s : b->s) s.fpv = like->s[0].fpv;
     acu::refresh_fp(b);
+    if (b->bp_ty != like->bp_ty) acu::set_bp_tile(b, like->bp_ty);
   }
   return b;
 }
@@ -1512,11 +1646,12 @@ void volume_run(acu_ctx* ctx, int nslices, int N, int T, int wpu, const float* t
     if (fbp) ACU_HIP_CHECK(hipHostMalloc((void**)&L.fbp_pin, (size_t)B * NN * 4, hipHostMallocDefault));
     if (tv) ACU_HIP_CHECK(hipHostMalloc((void**)&L.tv_pin, (size_t)B * NN * 4, hipHostMallocDefault));
   }
-  if (tune) {
-    (void)acu::autotune(lanes[0].full, 2);
+  if (tune) {  // an FBP-only call never runs the projector: its variants stay untimed
+    (void)acu::autotune(lanes[0].full, 2, tv ? acu::TUNE_ALL : acu::TUNE_RAMP | acu::TUNE_BP);
     lanes[1].full->ramp_variant = lanes[0].full->ramp_variant;
     for (auto& s : lanes[1].full->s) s.fpv = lanes[0].full->s[0].fpv;
     acu::refresh_fp(lanes[1].full);
+    acu::set_bp_tile(lanes[1].full, lanes[0].full->bp_ty);
   }
   const auto t1 = std::chrono::steady_clock::now();
   // results of a lane's last group: wait for its stream, then unstage

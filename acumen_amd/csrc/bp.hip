@@ -7,29 +7,41 @@
 // y2 < N+1.  Angles are summed in order, one pixel per accumulator.
 //
 // gfx950 mapping:
-//  * one workgroup = a 64 (x) x 32 (y) pixel tile, one lane per x column,
-//    each wave owns 8 rows, so one (sin, cos) pair and one LDS row serve
-//    8 updates per lane;
+//  * one workgroup = a 64 (x) x TY (y) pixel tile, TY = 16, 32 or 64 (a
+//    template parameter the autotuner picks per batch), one lane per x
+//    column; 4 waves (8 for TY = 64) own 8 rows each (4 for TY = 16), so
+//    one (sin, cos) pair and one LDS row serve 8 updates per lane;
 //  * angles are processed in blocks of 32; for each angle the tile only
-//    touches a window of <= 73 detector bins, which is staged in LDS as
-//    pairs P[t] = (S[lo+t], S[lo+t+1]-S[lo+t]) so an update is one 64-bit
-//    LDS read and one FMA:  S(y1) + frac*(S(y2)-S(y1));
+//    touches a window of W detector bins (72 / 80 / 96), which is staged in
+//    LDS as pairs P[t] = (S[lo+t], S[lo+t+1]-S[lo+t]) so an update is one
+//    64-bit LDS read and one FMA:  S(y1) + frac*(S(y2)-S(y1));
+//  * staging: 8 (16) threads per angle, each loading W/8 (W/16) pairs at
+//    clamped (always in-image) positions with every load issued before the
+//    first is used;
 //  * a pair whose lower tap y1 violates the reference's bounds test is
 //    staged as (0, 0), which reproduces the test exactly with no branch;
 //  * positions are taken relative to the tile origin: yr - 1 - lo =
-//    s*xl + c*yl + K with xl, yl < 64 and K = s*(x0+1-m) + c*(y0+1-m) + m-1-lo
-//    formed in FP64 once per angle and tile, so the FP32 rounding of a
-//    position is that of a number below ~140 instead of one near N (the
+//    s*xl + c*yl + K with xl < 64, yl < TY and K = s*(x0+1-m) + c*(y0+1-m)
+//    + m-1-lo formed in FP64 once per angle and tile, so the FP32 rounding
+//    of a position is that of a number below ~100 instead of one near N (the
 //    reference rounds s*(x-m) + c*(y-m) + m at magnitude N, ~1e-4 px at
 //    N = 2049); each 32-angle block is summed separately and then added;
-//  * tiles whose mask is identically zero only store zeros.
+//  * tiles whose mask is identically zero (by 64 x 16 sub-tiles) only store
+//    zeros.
 #include "acu_internal.hpp"
 
 namespace acu {
 namespace {
 
-constexpr int TX = BP_TX, TY = BP_TY, PY = BP_PY, AB = BP_AB, W = BP_W;
-constexpr int NWAVES = BP_THREADS / 64;
+constexpr int TX = BP_TX, AB = BP_AB;
+constexpr int bp_waves(int ty) { return ty == 64 ? 8 : 4; }
+
+// Detector bins staged per angle at tile height ty.  Over a 64 x ty tile the
+// positions span at most hypot(63, ty-1) bins; the window starts at least
+// one bin (less FP32 rounding) below the lowest position, and a sample at
+// local position t reads pair t, so W >= hypot(63, ty-1) + 3.01 suffices
+// (67.8, 73.2, 92.1); W is a multiple of the staging threads per angle.
+constexpr int bp_window(int ty) { return ty == 16 ? 72 : ty == 32 ? 80 : 96; }
 
 __device__ __forceinline__ int find_task(const BPTask* tasks, int ntasks, int gtile) {
   int lo = 0, hi = ntasks - 1;
@@ -40,7 +52,11 @@ __device__ __forceinline__ int find_task(const BPTask* tasks, int ntasks, int gt
   return lo;
 }
 
-__global__ __launch_bounds__(BP_THREADS) void bp_kernel(const BPTask* __restrict__ tasks, int ntasks, int base) {
+template <int TY>
+__global__ __launch_bounds__(64 * bp_waves(TY)) void bp_kernel(const BPTask* __restrict__ tasks, int ntasks, int base) {
+  constexpr int NW = bp_waves(TY), NT = 64 * NW, PY = TY / NW, W = bp_window(TY), SUBS = TY / BP_SUB;
+  constexpr int SPA = NT / AB, NST = W / SPA;  // staging threads per angle, pairs per thread
+  static_assert(TY % BP_SUB == 0 && TY % NW == 0 && NT % AB == 0 && W % SPA == 0, "tile shape");
   __shared__ float2 P[AB * W];
   __shared__ int lo_s[AB];
   __shared__ float k_s[AB];    // K of the tile-local position (see above)
@@ -50,14 +66,20 @@ __global__ __launch_bounds__(BP_THREADS) void bp_kernel(const BPTask* __restrict
   const int N = tk.N, T = tk.T;
   const float m = tk.mid;
   const int local = gtile - tk.tile_begin;
-  const int tx = local % tk.ntx, ty = local / tk.ntx;
-  const int x0 = tx * TX, y0 = ty * TY;
+  const int tx = local % tk.ntx, tyi = local / tk.ntx;
+  const int x0 = tx * TX, y0 = tyi * TY;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int x = x0 + lane;
   const int yb = y0 + wave * PY;
 
-  if (!tk.tile_active[local]) {
+  bool active = false;
+#pragma unroll
+  for (int j = 0; j < SUBS; ++j) {
+    const int r16 = tyi * SUBS + j;
+    if (r16 < tk.nsub) active = active || tk.tile_active[r16 * tk.ntx + tx] != 0;
+  }
+  if (!active) {
     if (x < N) {
 #pragma unroll
       for (int r = 0; r < PY; ++r)
@@ -81,6 +103,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_kernel(const BPTask* __restrict
 
   const ACU_GLOBAL float* __restrict__ S = as_global(tk.sino);
   const float2* __restrict__ sc = tk.sc;
+  const int sa = tid / SPA, sj = tid % SPA;  // staging: angle of the block, first bin
 
   for (int a0 = 0; a0 < T; a0 += AB) {
     const int na = min(AB, T - a0);
@@ -95,16 +118,21 @@ __global__ __launch_bounds__(BP_THREADS) void bp_kernel(const BPTask* __restrict
                          (double)lo);
     }
     __syncthreads();
-    for (int idx = tid; idx < na * W; idx += BP_THREADS) {
-      const int a = idx / W, t = idx - a * W;
-      const int d = lo_s[a] + t;
-      const ACU_GLOBAL float* row = S + (size_t)(a0 + a) * N;
-      float2 pv = make_float2(0.0f, 0.0f);
-      if (d >= 1 && d <= N - 2) {
-        const float s0 = row[d], s1 = row[d + 1];
-        pv = make_float2(s0, s1 - s0);
+    if (sa < na) {
+      const int lo = lo_s[sa];
+      const ACU_GLOBAL float* row = S + (size_t)(a0 + sa) * N;
+      float2 v[NST];
+#pragma unroll
+      for (int k = 0; k < NST; ++k) {
+        const int dc = min(max(lo + sj + SPA * k, 1), N - 2);  // N >= 3: row[dc + 1] is in the row
+        v[k] = make_float2(row[dc], row[dc + 1]);
       }
-      P[idx] = pv;
+#pragma unroll
+      for (int k = 0; k < NST; ++k) {
+        const int d = lo + sj + SPA * k;
+        P[sa * W + sj + SPA * k] = (d >= 1 && d <= N - 2) ? make_float2(v[k].x, v[k].y - v[k].x)
+                                                          : make_float2(0.0f, 0.0f);
+      }
     }
     __syncthreads();
     float blk[PY];
@@ -116,7 +144,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_kernel(const BPTask* __restrict
       const float2* Pa = P + a * W;
 #pragma unroll
       for (int r = 0; r < PY; ++r) {
-        const float yr = fmaf(v.y, yl[r], base);  // detector position - 1 - lo, in [1, 73)
+        const float yr = fmaf(v.y, yl[r], base);  // detector position - 1 - lo, in (0, W - 1)
         const int t = (int)yr;
         const float fr = __builtin_amdgcn_fractf(yr);
         const float2 pv = Pa[t];
@@ -143,11 +171,16 @@ __global__ __launch_bounds__(BP_THREADS) void bp_kernel(const BPTask* __restrict
 }  // namespace
 
 int bp_tiles_x(int N) { return (N + TX - 1) / TX; }
-int bp_tiles_y(int N) { return (N + TY - 1) / TY; }
+int bp_tiles_y(int N, int ty) { return (N + ty - 1) / ty; }
 
-void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, hipStream_t s) {
+void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, int ty, hipStream_t s) {
   if (tiles <= 0) return;
-  hipLaunchKernelGGL(bp_kernel, dim3(tiles), dim3(BP_THREADS), 0, s, d_tasks, ntasks, base);
+  switch (ty) {
+    case 16: hipLaunchKernelGGL(bp_kernel<16>, dim3(tiles), dim3(64 * bp_waves(16)), 0, s, d_tasks, ntasks, base); break;
+    case 32: hipLaunchKernelGGL(bp_kernel<32>, dim3(tiles), dim3(64 * bp_waves(32)), 0, s, d_tasks, ntasks, base); break;
+    case 64: hipLaunchKernelGGL(bp_kernel<64>, dim3(tiles), dim3(64 * bp_waves(64)), 0, s, d_tasks, ntasks, base); break;
+    default: throw Error("backprojection tile height must be 16, 32 or 64");
+  }
 }
 
 }  // namespace acu

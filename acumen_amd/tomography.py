@@ -161,11 +161,12 @@ class Batch:
     # ---- tuning ----
     def autotune(self, reps: int = 2) -> dict:
         """Time every forward-projection variant for every distinct slice
-        shape and keep the fastest per slice, and the ramp-filter arithmetics
-        that meet the accuracy bar on the whole batch; returns the report
-        (ms per radon per slice and per batch ramp filter, and the choices)."""
+        shape and keep the fastest per slice, the ramp-filter arithmetics
+        that meet the accuracy bar and the backprojection tile heights on the
+        whole batch; returns the report (ms per radon per slice, per batch
+        ramp filter and per batch backprojection, and the choices)."""
         import json
-        cap = 1024 + 192 * len(self.shapes)
+        cap = 1536 + 192 * len(self.shapes)
         buf = ctypes.create_string_buffer(cap)
         check(self._lib.acu_batch_autotune(self.handle, int(reps), buf, cap))
         return json.loads(buf.value.decode())
@@ -191,6 +192,16 @@ class Batch:
 
     def set_fp_variant(self, i: int, variant: int) -> None:
         check(self._lib.acu_batch_set_fp_variant(self.handle, i, int(variant)))
+
+    def bp_tile(self) -> int:
+        """Backprojection tile height of the batch: 16, 32 or 64 rows."""
+        v = self._lib.acu_batch_bp_tile(self.handle)
+        if v < 0:
+            check(-v)
+        return v
+
+    def set_bp_tile(self, rows: int) -> None:
+        check(self._lib.acu_batch_set_bp_tile(self.handle, int(rows)))
 
     # ---- differentiation (no counterpart in the reference; built on radon) ----
     def radon_adjoint(self, sino) -> np.ndarray:

@@ -142,6 +142,7 @@ def run_mixed(args) -> int:
         for sl in rep["slices"]:
             choices[sl["choice"]] = choices.get(sl["choice"], 0) + 1
         choices["ramp"] = rep["ramp"]["choice"]
+        choices["bp_tile"] = rep["bp"]["choice"]
     for w in range(args.warmup):
         log(f"warmup step {w + 1}/{args.warmup}")
         batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_GRAPH)
@@ -545,13 +546,15 @@ def main() -> int:
     ctx.synchronize()
     tuning = None
     if not args.no_tune:
-        log("autotune: projector variant per slice shape, ramp-filter arithmetic")
+        log("autotune: projector variant per slice shape, ramp-filter arithmetic, backprojection tile")
         t = time.perf_counter()
         rep = batch.autotune(reps=2)
         tuning = {"seconds": round(time.perf_counter() - t, 2),
                   "fp_variant": rep["slices"][0]["choice"], "ramp": rep["ramp"]["choice"],
+                  "bp_tile": rep["bp"]["choice"],
                   "ramp_ms_per_batch": {k: round(v, 4) for k, v in rep["ramp"]["ms"].items()},
-                  "fp_ms_per_slice": {k: round(v, 4) for k, v in rep["slices"][0]["ms"].items()}}
+                  "fp_ms_per_slice": {k: round(v, 4) for k, v in rep["slices"][0]["ms"].items()},
+                  "bp_ms_per_batch": {k: round(v, 4) for k, v in rep["bp"]["ms"].items()}}
 
     for w in range(args.warmup):
         log(f"warmup step {w + 1}/{args.warmup}")

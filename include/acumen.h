@@ -150,8 +150,10 @@ int acu_tvopt(acu_batch *b, const float *f_host, float mu, float lambda, int nou
  * distinct slice shape (N, T, angles) and keeps the fastest for all later
  * calls.  It also times the ramp-filter arithmetics that meet the FBP and
  * TV accuracy bar (ACU_RAMP_FP64, _FP32, _BF16X6, _FFT; not _BF16X3) on the
- * whole batch and keeps the fastest.  report (may be NULL) receives a JSON summary of the timings and
- * choices, truncated to cap bytes including the terminating NUL.  The
+ * whole batch and keeps the fastest, and the backprojection tile height
+ * (below) on the whole batch.  report (may be NULL) receives a JSON summary
+ * of the timings and choices, truncated to cap bytes including the
+ * terminating NUL.  The
  * getter returns the variant, or -ACU_ERR_ARG for a bad slice; the setter
  * lets a caller reapply a stored choice. */
 int acu_batch_autotune(acu_batch *b, int reps, char *report, int cap);
@@ -173,6 +175,15 @@ int acu_batch_set_ramp_variant(acu_batch *b, int variant);
 int acu_batch_ramp_variant(const acu_batch *b);
 int acu_batch_fp_variant(const acu_batch *b, int slice);
 int acu_batch_set_fp_variant(acu_batch *b, int slice, int variant);
+
+/* Backprojection tile of the batch (tv_hpc.F90:107-123 is its arithmetic):
+ * 64-pixel-wide tiles of 16, 32 (default) or 64 rows, one workgroup each.
+ * The angle blocks (32 angles, each block summed separately) are the same
+ * at every height; positions are rounded relative to the tile origin, so
+ * heights differ by FP32 rounding only.  The initial value comes from
+ * ACUMEN_BP_TILE.  The getter returns the rows or -ACU_ERR_ARG. */
+int acu_batch_set_bp_tile(acu_batch *b, int rows);
+int acu_batch_bp_tile(const acu_batch *b);
 
 /* ---- Differentiation.  No counterpart in the reference: these build on
  * its projector (tv_hpc.F90:129-159) and the periodic differences of its

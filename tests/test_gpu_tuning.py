@@ -1,7 +1,8 @@
-"""Forward-projection variants and the autotuner on the GPU: every variant
-matches the oracle, a batch whose slices use different variants (one launch
-per variant group) equals single-slice plans bit for bit, and autotuning
-keeps results within the parity bar."""
+"""Forward-projection variants, backprojection tile heights and the
+autotuner on the GPU: every variant and tile height matches the oracle, a
+batch whose slices use different variants (one launch per variant group)
+equals single-slice plans bit for bit, and autotuning keeps results within
+the parity bar."""
 import numpy as np
 import pytest
 
@@ -48,6 +49,67 @@ def test_cell_boxes_equal_raw_boxes(ctx, N, T):
         b.set_fp_variant(0, 8)
         cell = b.radon(x)
     assert np.array_equal(raw.view(np.uint32), cell.view(np.uint32))
+
+
+@pytest.mark.parametrize("N,T", [(65, 48), (257, 180), (1025, 60), (129, 7), (2049, 40)])
+def test_every_bp_tile_matches_oracle(ctx, N, T):
+    """iradon of a noisy sinogram at tile heights 16, 32, 64 against the FP64
+    oracle: within 1e-5, or no further than the reference's own FP32
+    arithmetic is where the noise takes that past 1e-5 (1025 x 60: 4.9e-5).
+    The heights differ only in where FP32 positions are rounded (relative to
+    the tile origin): they agree with each other within 1e-5."""
+    th = synth.angles(T)
+    rng = np.random.default_rng(N + T)
+    s = (ref.radon(disk_phantom(N, 8, N + 2 * T), th, "f64")
+         + rng.normal(0.0, 0.05, (T, N))).astype(np.float32)
+    w = synth.default_wpu(N)
+    want = ref.iradon(s, th, w, "f64")
+    bar = max(FBP_TOL, rel(ref.iradon(s, th, w, "f32"), want))
+    got = {}
+    with Batch(ctx, [(N, T, w, th)]) as b:
+        assert b.bp_tile() == 32
+        for ty in (16, 32, 64):
+            say("bp tile", ty, N, T)
+            b.set_bp_tile(ty)
+            assert b.bp_tile() == ty
+            got[ty] = b.iradon(s)
+            say("tile", ty, "vs FP64:", rel(got[ty], want), "bar:", bar)
+            assert rel(got[ty], want) < bar, (ty, rel(got[ty], want), bar)
+        with pytest.raises(AcumenError):
+            b.set_bp_tile(48)
+        assert b.bp_tile() == 64
+    for ty in (16, 64):
+        say("tile", ty, "vs 32:", rel(got[ty], got[32]))
+        assert rel(got[ty], got[32]) < FBP_TOL, (ty, rel(got[ty], got[32]))
+
+
+@pytest.mark.parametrize("ty", [16, 64])
+def test_bp_tile_batch_equals_single_plans(ctx, ty):
+    """A batch at one tile height (graph-replayed TV) equals single-slice
+    plans at the same height bit for bit."""
+    shapes = [(65, 48), (129, 37), (97, 100)]
+    say("bp tile batch", ty, shapes)
+    rng = np.random.default_rng(ty)
+    items = []
+    for i, (N, T) in enumerate(shapes):
+        th = np.sort(rng.uniform(0, 180, T)).astype(np.float32)
+        f = ref.radon(disk_phantom(N, 5, 80 + i), th, "f64").astype(np.float32)
+        items.append((N, T, synth.default_wpu(N), th, f))
+    with Batch(ctx, [(N, T, w, th) for N, T, w, th, _ in items]) as b:
+        for i, it in enumerate(items):
+            b.set_sino(i, it[4])
+        b.tvopt(0.05, 0.1, 1, 1, 2, ACU_TV_GRAPH)   # a captured graph at the default height
+        b.set_bp_tile(ty)                            # is rebuilt at the new one
+        b.tvopt(0.05, 0.1, 3, 1, 4, ACU_TV_GRAPH)
+        batched = [b.get_image(i, ACU_IMG_TV) for i in range(len(items))]
+    for i, (N, T, w, th, f) in enumerate(items):
+        with Batch(ctx, [(N, T, w, th)]) as p:
+            p.set_bp_tile(ty)
+            p.set_sino(0, f)
+            p.tvopt(0.05, 0.1, 3, 1, 4, ACU_TV_PLAIN)
+            assert np.array_equal(p.get_image(0, ACU_IMG_TV), batched[i]), i
+        tv64, _, _ = ref.tvopt(f, th, w, 0.05, 0.1, 3, 1, 4, "f64")
+        assert rel(batched[i], tv64) < TV_TOL
 
 
 def test_mixed_variant_batch_equals_single_plans(ctx):
@@ -104,6 +166,11 @@ def test_autotune_report_and_results(ctx):
         rnames = {_lib.RAMP_FP64: "fp64", _lib.RAMP_FP32: "fp32", _lib.RAMP_BF16X6: "bf16x6",
                   _lib.RAMP_FFT: "fft"}
         assert rnames[b.ramp_variant()] == rmp["choice"]
+        # backprojection tile height: fastest over the batch
+        bpr = rep["bp"]
+        assert set(bpr["ms"]) == {"16", "32", "64"}
+        assert str(bpr["choice"]) == min(bpr["ms"], key=bpr["ms"].get)
+        assert b.bp_tile() == bpr["choice"]
         b.tvopt(0.05, 0.1, 2, 1, 3, ACU_TV_GRAPH)
         for i, (N, T, w, th, f) in enumerate(items):
             tv64, _, _ = ref.tvopt(f, th, w, 0.05, 0.1, 2, 1, 3, "f64")
