@@ -145,9 +145,11 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
           bxl[j] = act[j] ? (m - sy[j]) - (float)bx0 : 1.0f;
           byl[j] = act[j] ? (m + cyv[j]) - (float)by0 : 1.0f;
         }
+        // u = x - m is an integer below 2^24 in magnitude, so stepping it by
+        // 1.0f gives exactly (float)x - m without a conversion per step
+        float u = (float)c0 - m;
 #pragma unroll 2
-        for (int x = c0; x <= c1; ++x) {
-          const float u = (float)x - m;
+        for (int x = c0; x <= c1; ++x, u += 1.0f) {
           float val[AL];
 #pragma unroll
           for (int j = 0; j < AL; ++j) {
