@@ -1598,12 +1598,14 @@ void check_rc(int rc) {
 }
 
 // Host copy between caller memory and pinned staging, split over
-// ACUMEN_COPY_THREADS threads (default 8): one thread's memcpy reaches only a
-// fraction of the host memory bandwidth that the PCIe transfers need.
+// ACUMEN_COPY_THREADS threads (default: the host's hardware threads, at most
+// 16): one thread's memcpy reaches only a fraction of the host memory
+// bandwidth that the PCIe transfers need.
 void par_copy(void* dst, const void* src, size_t bytes) {
   static const int nt = [] {
     const char* e = std::getenv("ACUMEN_COPY_THREADS");
-    const int v = e ? std::atoi(e) : 8;
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int v = e ? std::atoi(e) : std::min(16, hw > 0 ? hw : 8);
     return v < 1 ? 1 : v > 64 ? 64 : v;
   }();
   const size_t chunk = (bytes + nt - 1) / nt;

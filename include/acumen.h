@@ -237,8 +237,8 @@ int acu_batch_loss_grad_dev(acu_batch *b, const float *const *x_dev, const float
  * second over the call, [2] milliseconds of set-up (lanes, pinned buffers,
  * tuning), [3] slices per second of the streaming after set-up; PCIe and
  * host staging are included in all.  Synchronous; uses device memory for 2
- * batches.  ACUMEN_COPY_THREADS (default 8) sets the host threads of the
- * staging copies. */
+ * batches.  ACUMEN_COPY_THREADS (default: the host's hardware threads, at
+ * most 16) sets the host threads of the staging copies. */
 int acu_volume_reconstruct(acu_ctx *ctx, int nslices, int nimage, int ntheta, int wpu,
                            const float *theta_deg, const float *sino_host, float *fbp_host,
                            float *tv_host, float mu, float lambda, int nouter, int ninner, int ncg,
