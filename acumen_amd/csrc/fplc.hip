@@ -33,6 +33,8 @@
 // global cells of fp.hip.
 #include "fpl_common.hpp"
 
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 
@@ -385,11 +387,13 @@ void launch_forward_cell(const FPLTask* d_tasks, int ntasks, int base, int block
   if (blocks <= 0) return;
   int dev = 0;
   ACU_HIP_CHECK(hipGetDevice(&dev));
-  static bool attr_set[64] = {};
-  if (dev < 64 && !attr_set[dev]) {
+  // the LDS size above 64 KB needs the attribute once per device; setting it
+  // is idempotent, so threads that race here at most set it twice
+  static std::atomic<bool> attr_set[64];
+  if (dev >= 64 || !attr_set[dev].load(std::memory_order_acquire)) {
     ACU_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&fplc_kernel),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, C_LDS));
-    attr_set[dev] = true;
+    if (dev < 64) attr_set[dev].store(true, std::memory_order_release);
   }
   hipLaunchKernelGGL(fplc_kernel, dim3(xcd_grid(blocks)), dim3(C_THREADS), C_LDS, s, d_tasks, ntasks, base, blocks);
 }

@@ -13,12 +13,14 @@ reduction returned to the host before the next kernel, as the reference is
 structured) and FBP slices/s of the batched path.
 
 roofline: the forward projector dominates TV time (6 of the 13 projector
-launches per outer iteration and ~85% of the time); its algorithmic work is
+launches per outer iteration and ~74% of the time); its algorithmic work is
 14 FP32 flops per ray sample (2 coordinates, 2 fractions, 3 lerps, 1
 accumulate) over N*N*T samples per slice, measured with HIP events on the
-library's stream during the timed steps.  The kernel is bound by its LDS
-gathers rather than the VALU (16 bytes of staged box per sample), so
-`roofline.lds` reports that rate as well.
+library's stream during the timed steps.  The kernel is co-limited by the
+VALU issue rate and its LDS gathers (16 bytes of staged box per sample), so
+`roofline.lds` reports that rate as well.  `roofline_backprojection` gives
+the same figures for the backprojector (6 flop per pixel-angle update over
+N*N*T updates, compulsory bytes 4(N*T + 2*N*N) per slice).
 
 cpu_baseline: the oracle's FP32 restatement of the reference's loops, one
 slice per host thread as the reference's MPI ranks do, on a bounded sample
@@ -41,6 +43,7 @@ N_DET = 2048
 T_ANG = 1500
 MU, LAM, NOUTER, NINNER, NCG = 0.05, 0.1, 50, 1, 5
 FP_FLOPS_PER_SAMPLE = 14
+BP_FLOPS_PER_UPDATE = 6  # coordinate (2), fraction, lerp (2), accumulate
 FP32_PEAK_TFLOPS = 157.3
 LDS_BYTES_PER_SAMPLE = 16  # the 2x2 cell of a bilinear sample
 N_CU, CLOCK_GHZ = 256, 2.4
@@ -628,6 +631,22 @@ def main() -> int:
                 "B_per_clk_per_cu": round(lds_bytes / (N_CU * CLOCK_GHZ * 1e9), 1)},
     }
 
+    bp_ms, bp_n = kt["bp"]
+    bp_avg_s = bp_ms / max(bp_n, 1) * 1e-3
+    bp_achieved = BP_FLOPS_PER_UPDATE * samples_per_launch / bp_avg_s / 1e12
+    roofline_bp = {
+        "bound": "valu",
+        "kernel": "bp_kernel (pixel-driven linear-interpolation backprojection on the FP32 VALU)",
+        "achieved": round(bp_achieved, 3),
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(bp_achieved / FP32_PEAK_TFLOPS, 4),
+        "avg_launch_ms": round(bp_avg_s * 1e3, 4),
+        "launches": bp_n,
+        "work_per_launch": f"{BP_FLOPS_PER_UPDATE} flop x {N}*{N}*{T} updates x {B} slices",
+        "compulsory_hbm_GBps": round(4.0 * (N * T + 2.0 * N * N) * B / bp_avg_s / 1e9, 1),
+    }
+
     cpu = None
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         log(f"cpu baseline: {args.cpu_threads} threads")
@@ -658,6 +677,7 @@ def main() -> int:
         "fbp_slices_per_s": round(fbp_rate, 3),
         "kernel_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
         "roofline": roofline,
+        "roofline_backprojection": roofline_bp,
         "cpu_baseline": cpu,
         "autotune": tuning,
     }
