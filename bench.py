@@ -593,18 +593,22 @@ def main() -> int:
     ctx.synchronize()
     fbp_rate = B * reps / (time.perf_counter() - t1)
 
-    plain_rate = None
+    plain_rate, plain_equal = None, None
     if not args.no_plain:
         log("plain per-slice path: 1 slice")
         with Batch(ctx, [(N, T, wpu, th)]) as one:
             one.set_ramp_variant(batch.ramp_variant())
             one.set_fp_variant(0, batch.fp_variant(0))
+            one.set_bp_tile(batch.bp_tile())
             one.set_sino(0, sinos[0])
             ctx.synchronize()
             t2 = time.perf_counter()
             one.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_PLAIN)
             ctx.synchronize()
             plain_rate = 1.0 / (time.perf_counter() - t2)
+            # full-size property: the batched path's slice 0 is the plain path's, bit for bit
+            plain_equal = bool(np.array_equal(one.get_image(0, ACU_IMG_TV).view(np.uint32),
+                                              tv_img.view(np.uint32)))
     batch.close()
     ctx.close()
 
@@ -674,6 +678,7 @@ def main() -> int:
                    "ncg": NCG, "wpu": wpu},
         "plain_slices_per_s": None if plain_rate is None else round(plain_rate, 5),
         "batched_vs_plain": None if plain_rate is None else round(value / plain_rate, 3),
+        "tv_batched_equals_plain_bitwise": plain_equal,
         "fbp_slices_per_s": round(fbp_rate, 3),
         "kernel_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
         "roofline": roofline,
