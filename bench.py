@@ -157,7 +157,13 @@ def run_mixed(args) -> int:
         log(f"step {k + 1}/{args.steps} done at {time.perf_counter() - t0:.2f} s")
     elapsed = time.perf_counter() - t0
     value = len(specs) * args.steps / elapsed
-    assert all(np.all(np.isfinite(batch.get_image(i, ACU_IMG_TV))) for i in (0, len(specs) - 1))
+    graph_imgs = [batch.get_image(i, ACU_IMG_TV).view(np.uint32).copy() for i in range(len(specs))]
+    assert all(np.all(np.isfinite(g.view(np.float32))) for g in graph_imgs)
+
+    def same_as_graph() -> bool:
+        return all(np.array_equal(batch.get_image(i, ACU_IMG_TV).view(np.uint32), g)
+                   for i, g in enumerate(graph_imgs))
+
     # kernel breakdown of one batched step (events; graph replay is off while timing)
     ctx.set_timing(True)
     batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_BATCHED)
@@ -166,6 +172,8 @@ def run_mixed(args) -> int:
           (("ramp", _lib.KIND_RAMP), ("bp", _lib.KIND_BP), ("cells", _lib.KIND_CELLS),
            ("fp", _lib.KIND_FP))}
     ctx.set_timing(False)
+    # full-size property: the three modes give the same bits on every slice
+    modes_equal = {"batched": same_as_graph()}
     plain_rate = None
     if not args.no_plain:
         log("plain per-slice path: all slices, one after another")
@@ -174,6 +182,7 @@ def run_mixed(args) -> int:
         batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_PLAIN)
         ctx.synchronize()
         plain_rate = len(specs) / (time.perf_counter() - t2)
+        modes_equal["plain"] = same_as_graph()
     batch.close()
     ctx.close()
     widths = {}
@@ -200,6 +209,7 @@ def run_mixed(args) -> int:
                    "total_N2T": float(sum(sp["N"] ** 2 * sp["T"] for sp in specs))},
         "plain_slices_per_s": None if plain_rate is None else round(plain_rate, 5),
         "batched_vs_plain": None if plain_rate is None else round(value / plain_rate, 3),
+        "tv_equals_graph_bitwise": modes_equal,
         "kernel_ms_per_step": {k: round(v[0], 2) for k, v in kt.items()},
         "autotune_s": None if tune_s is None else round(tune_s, 2),
         "fp_variant_counts": choices,
