@@ -163,8 +163,9 @@ def test_full_size_fbp_and_radon(ctx):
 def test_full_size_tv_against_fp64(ctx):
     """The production slice (2049^2 x 1500, 40-ellipse phantom, Poisson
     I0 = 5e3): split-Bregman TV, 1 outer x 2 CG iterations, against the FP64
-    oracle -- with the library defaults and with the autotuner's choices at
-    this shape (LDS projector with 4 angles per lane, FFT ramp filter)."""
+    oracle -- with the library defaults, with the autotuner's choices at
+    this shape (LDS projector with 4 angles per lane, FFT ramp filter), with
+    the cell-box projector and with backprojection tiles of 16 and 64 rows."""
     N, T = 2049, 1500
     say("full-size TV", N, T)
     th = synth.angles(T)
@@ -183,6 +184,11 @@ def test_full_size_tv_against_fp64(ctx):
         b.set_fp_variant(0, 8)  # cell-box projector
         b.tvopt(0.05, 0.1, 1, 1, 2, ACU_TV_GRAPH)
         results["cell16"] = b.get_image(0, ACU_IMG_TV)
+        b.set_fp_variant(0, 4)
+        for ty in (16, 64):  # backprojection tile heights
+            b.set_bp_tile(ty)
+            b.tvopt(0.05, 0.1, 1, 1, 2, ACU_TV_GRAPH)
+            results[f"bp_tile{ty}"] = b.get_image(0, ACU_IMG_TV)
     ref.set_threads(16)
     try:
         tv64, _, _ = ref.tvopt(f, th, wpu, 0.05, 0.1, 1, 1, 2, "f64")
