@@ -63,6 +63,16 @@ __global__ __launch_bounds__(ADJ_T * ADJ_T) void adjoint_kernel(const AdjTask* _
   if (px > N || py > N) return;  // no barrier below
   const float X = (float)px - m, Y = (float)py - m;
   const ACU_GLOBAL float* __restrict__ sino = as_global(tk.sino);
+  // Pixels with 3 <= px, py <= N-1: the cells x1 in {px-1, px} that can
+  // cover them always pass 2 <= x1 <= N-1, and the projector's weights
+  // (x1+1) - xr and xr - x1 are differences of floats within a factor of two
+  // of each other, hence exact (Sterbenz), and equal to 1 - |xr - px|, which
+  // is exact too (xr - px is, and 1 - d is a multiple of xr's ulp below 1).
+  // So the test and weights of those pixels reduce to the two weights being
+  // positive (a weight of 0 adds nothing) -- the same bits with no floor,
+  // conversion or select.  The rim keeps the projector's own test.
+  const bool inner = px >= 3 && px <= N - 1 && py >= 3 && py <= N - 1;
+  const float pxf = (float)px, pyf = (float)py;
   float acc = 0.0f;
   for (int t0 = 0; t0 < T; t0 += ADJ_AB) {
     const int t1 = min(T, t0 + ADJ_AB);
@@ -77,27 +87,56 @@ __global__ __launch_bounds__(ADJ_T * ADJ_T) void adjoint_kernel(const AdjTask* _
       const float hw = fabsf(c) + fabsf(s) + 1e-3f;
       const int jx0 = (int)ceilf(qx - hw), jy0 = (int)ceilf(qy - hw);
       const ACU_GLOBAL float* row = sino + (size_t)t * N;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const int jx = jx0 + a;
-        const float u = (float)jx - m;
-        const float cu = c * u, su = s * u;
+      if (inner) {
+        float sw[3], cw[3], rv[3];
+        bool yok[3];
 #pragma unroll
         for (int bb = 0; bb < 3; ++bb) {
           const int jy = jy0 + bb;
           const float w = (float)jy - m;
-          const float xr = (cu - s * w) + m;
-          const float yr = (su + c * w) + m;
-          const float fx = floorf(xr), fy = floorf(yr);
-          const int x1 = (int)fx, y1 = (int)fy;
-          const int dx = px - x1, dy = py - y1;
-          const bool hit = jx >= 1 && jx <= N && jy >= 1 && jy <= N && (unsigned)dx <= 1u &&
-                           (unsigned)dy <= 1u && x1 >= 2 && x1 <= N - 1 && y1 >= 2 && y1 <= N - 1;
-          if (hit) {
-            // the projector's weights: (x2 - xr) for x1, (xr - x1) for x2
-            const float wx = dx == 0 ? (fx + 1.0f) - xr : xr - fx;
-            const float wy = dy == 0 ? (fy + 1.0f) - yr : yr - fy;
-            blk += (row[jy - 1] * wx) * wy;
+          sw[bb] = s * w;
+          cw[bb] = c * w;
+          yok[bb] = jy >= 1 && jy <= N;
+          rv[bb] = yok[bb] ? row[jy - 1] : 0.0f;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const int jx = jx0 + a;
+          const float u = (float)jx - m;
+          const float cu = c * u, su = s * u;
+          const bool xok = jx >= 1 && jx <= N;
+#pragma unroll
+          for (int bb = 0; bb < 3; ++bb) {
+            const float xr = (cu - sw[bb]) + m;
+            const float yr = (su + cw[bb]) + m;
+            const float wx = 1.0f - fabsf(xr - pxf);
+            const float wy = 1.0f - fabsf(yr - pyf);
+            if (xok && yok[bb] && wx > 0.0f && wy > 0.0f) blk += (rv[bb] * wx) * wy;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const int jx = jx0 + a;
+          const float u = (float)jx - m;
+          const float cu = c * u, su = s * u;
+#pragma unroll
+          for (int bb = 0; bb < 3; ++bb) {
+            const int jy = jy0 + bb;
+            const float w = (float)jy - m;
+            const float xr = (cu - s * w) + m;
+            const float yr = (su + c * w) + m;
+            const float fx = floorf(xr), fy = floorf(yr);
+            const int x1 = (int)fx, y1 = (int)fy;
+            const int dx = px - x1, dy = py - y1;
+            const bool hit = jx >= 1 && jx <= N && jy >= 1 && jy <= N && (unsigned)dx <= 1u &&
+                             (unsigned)dy <= 1u && x1 >= 2 && x1 <= N - 1 && y1 >= 2 && y1 <= N - 1;
+            if (hit) {
+              // the projector's weights: (x2 - xr) for x1, (xr - x1) for x2
+              const float wx = dx == 0 ? (fx + 1.0f) - xr : xr - fx;
+              const float wy = dy == 0 ? (fy + 1.0f) - yr : yr - fy;
+              blk += (row[jy - 1] * wx) * wy;
+            }
           }
         }
       }

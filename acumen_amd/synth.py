@@ -96,6 +96,24 @@ def poisson_noise(sino: np.ndarray, I0: float, N: int, seed: int) -> np.ndarray:
     return (-np.log(np.maximum(c, 1.0) / I0) / N).astype(np.float32)
 
 
+def shift_rows(sino: np.ndarray, shifts: np.ndarray) -> np.ndarray:
+    """A centre-of-rotation error per projection: row t of the (T, N)
+    sinogram sampled at detector position d - shifts[t] by linear
+    interpolation, zero off the detector (FP64 arithmetic, stored FP32)."""
+    T, N = sino.shape
+    s = sino.astype(np.float64)
+    d = np.arange(N, dtype=np.float64)[None, :] - np.asarray(shifts, np.float64)[:, None]
+    i0 = np.floor(d).astype(np.int64)
+    w = d - i0
+    rows = np.arange(T)[:, None]
+
+    def at(i):
+        ok = (i >= 0) & (i < N)
+        return np.where(ok, s[rows, np.clip(i, 0, N - 1)], 0.0)
+
+    return ((1.0 - w) * at(i0) + w * at(i0 + 1)).astype(np.float32)
+
+
 MIXED_WIDTHS = (512, 768, 1024, 1536, 2048)
 
 

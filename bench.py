@@ -44,6 +44,7 @@ T_ANG = 1500
 MU, LAM, NOUTER, NINNER, NCG = 0.05, 0.1, 50, 1, 5
 FP_FLOPS_PER_SAMPLE = 14
 BP_FLOPS_PER_UPDATE = 6  # coordinate (2), fraction, lerp (2), accumulate
+GRAD_LAM, GRAD_EPS = 0.1, 1e-3  # differentiation workload: TV weight and smoothing
 FP32_PEAK_TFLOPS = 157.3
 LDS_BYTES_PER_SAMPLE = 16  # the 2x2 cell of a bilinear sample
 N_CU, CLOCK_GHZ = 256, 2.4
@@ -502,6 +503,163 @@ def run_ingest(args) -> int:
     return 0
 
 
+class DeviceBuffers:
+    """Device memory from libamdhip64, the HIP runtime the library uses, for
+    the acu_*_dev entry points."""
+
+    def __init__(self):
+        import ctypes
+        self.ct = ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.hip.hipFree.argtypes = [ctypes.c_void_p]
+        self.hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.ptrs = []
+
+    def alloc(self, nbytes):
+        p = self.ct.c_void_p()
+        if self.hip.hipMalloc(self.ct.byref(p), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+        self.ptrs.append(p.value)
+        return p.value
+
+    def upload(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        if self.hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) != 0:  # host to device
+            raise RuntimeError("hipMemcpy failed")
+        return p
+
+    def close(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+        self.ptrs = []
+
+
+def grad_cpu_baseline(x, th, sh, b, threads):
+    """The oracle's FP32 loss + gradients of one slice on `threads` threads."""
+    from oracle import ref
+    ref.set_threads(threads)
+    try:
+        t = time.perf_counter()
+        ref.loss_grad(x, th, sh, b, GRAD_LAM, GRAD_EPS, "f32")
+        wall = time.perf_counter() - t
+    finally:
+        ref.set_threads(1)
+    return {"value": round(1.0 / wall, 4), "unit": "slice evaluations/s", "cores": threads, "kind": "port",
+            "sample": f"one slice's loss, dL/dx and dL/ds by the oracle's FP32 loops on {threads} threads, "
+                      f"{wall:.2f} s"}
+
+
+def run_grad(args) -> int:
+    """Differentiation: 16 slices of 512^2 (N = 513) x 720 angles whose
+    sinograms carry per-angle centre-of-rotation shifts drawn N(0, 1.5 px),
+    applied by linear interpolation along the detector.  value: slice
+    evaluations/s of L = 1/2 |shift_s(radon(x)) - b|^2 + lambda TV_eps(x)
+    with dL/dx and dL/ds (lambda = 0.1, eps = 1e-3) for all 16 slices in one
+    batched call, images, shifts and gradients resident in device memory.
+    Also: the same with host buffers (PCIe included), and Gauss-Newton
+    registration of the shifts from zero (evaluations, time, error against
+    the drawn shifts).  cpu_baseline: the oracle's FP32 evaluation of one
+    slice."""
+    import ctypes
+
+    import numpy as np
+
+    from acumen_amd import Batch, Context, synth
+
+    N, T, S = synth.padded_size(512), 720, 16
+    th = synth.angles(T)
+    wpu = synth.default_wpu(N)
+    rng = np.random.default_rng(1003)
+    ctx = Context(0)
+    log(f"inputs: {S} slices of {N}x{N}, {T} angles, shifts N(0, 1.5 px)")
+    xs, bs, truth = [], [], []
+    with Batch(ctx, [(N, T, wpu, th)]) as p:
+        for i in range(S):
+            x = synth.phantom(N, 20, 1300 + i)
+            sh = rng.normal(0.0, 1.5, T).astype(np.float32)
+            xs.append(x)
+            truth.append(sh)
+            bs.append(synth.shift_rows(p.radon(x), sh))
+    b = Batch(ctx, [(N, T, wpu, th)] * S)
+    for i in range(S):
+        b.set_sino(i, bs[i])
+    sh0 = [t + np.float32(0.3) for t in truth]  # evaluated away from the optimum
+    steps = max(args.steps, 5)
+    log("loss + gradients, host buffers")
+    b.loss_grad(xs, sh0, GRAD_LAM, GRAD_EPS)
+    t = time.perf_counter()
+    for _ in range(steps):
+        loss_h, gx_h, gs_h = b.loss_grad(xs, sh0, GRAD_LAM, GRAD_EPS)
+    host_rate = S * steps / (time.perf_counter() - t)
+    log("loss + gradients, device buffers")
+    db = DeviceBuffers()
+    vp = ctypes.c_void_p
+    try:
+        d_x = (vp * S)(*[db.upload(x) for x in xs])
+        d_s = (vp * S)(*[db.upload(s) for s in sh0])
+        d_gx = (vp * S)(*[db.alloc(N * N * 4) for _ in range(S)])
+        d_gs = (vp * S)(*[db.alloc(T * 4) for _ in range(S)])
+        loss = np.zeros((S, 3), np.float64)
+        lp = loss.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+        def dev_eval():
+            rc = b._lib.acu_batch_loss_grad_dev(b.handle, d_x, d_s, GRAD_LAM, GRAD_EPS, d_gx, d_gs, None, lp)
+            if rc != 0:
+                raise RuntimeError(b._lib.acu_last_error().decode())
+
+        for _ in range(args.warmup):
+            dev_eval()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            dev_eval()
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t
+        dev_equal = bool(np.array_equal(loss, loss_h))
+    finally:
+        db.close()
+    log("registration from zero shifts")
+    t = time.perf_counter()
+    fit, evals = b.register_shifts(xs, iters=12)
+    t_reg = time.perf_counter() - t
+    err = np.concatenate([f - s for f, s in zip(fit, truth)])
+    b.close()
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline: the oracle's evaluation of one slice")
+        cpu = grad_cpu_baseline(xs[0], th, sh0[0], bs[0], args.cpu_threads)
+    ctx.close()
+    out = {
+        "metric": "slice evaluations/s of the TV-regularised loss with gradients in the image and the "
+                  "per-angle shifts, 512^2 x 720, 16 slices per call",
+        "value": round(S * steps / elapsed, 2),
+        "unit": "slice evaluations/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{S} slices of 512^2 (N={N}) x {T} angles, 20-ellipse phantoms, shifts "
+                               f"N(0, 1.5 px) (seed 1003), lambda={GRAD_LAM}, eps={GRAD_EPS}",
+                   "slices": S, "nimage": N, "ntheta": T},
+        "host_buffers_slice_evaluations_per_s": round(host_rate, 2),
+        "device_loss_equals_host_loss": dev_equal,
+        "registration": {"evaluations": evals, "seconds": round(t_reg, 3),
+                         "max_error_px": float(f"{np.abs(err).max():.3e}"),
+                         "rms_error_px": float(f"{np.sqrt(np.mean(err ** 2)):.3e}")},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -514,11 +672,13 @@ def main() -> int:
     ap.add_argument("--nouter", type=int, default=NOUTER,
                     help="outer iterations (default: the workload's 50; fewer only for profiling runs)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
-    ap.add_argument("--workload", choices=("production", "mixed", "volume", "prep", "ingest"), default="production",
+    ap.add_argument("--workload", choices=("production", "mixed", "volume", "prep", "ingest", "grad"),
+                    default="production",
                     help="production: 2048^2 x 1500 slices (the headline); mixed: 64 slices of "
                          "uneven width and angle count in one batch; volume: a 2048^2 x 1500 x 256 "
                          "volume streamed through FBP (+ TV on --tv-slices slices); prep: projection -> "
-                         "sinogram preprocessing; ingest: FITS ingest normalisation")
+                         "sinogram preprocessing; ingest: FITS ingest normalisation; grad: loss and "
+                         "gradients for shift registration")
     ap.add_argument("--slices", type=int, default=256, help="volume workload: slices streamed through FBP")
     ap.add_argument("--tv-slices", type=int, default=16,
                     help="volume workload: slices streamed through FBP + 30-iteration TV")
@@ -527,7 +687,7 @@ def main() -> int:
         args.steps = 50 if args.workload == "ingest" else 2
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(slices are independent; multi-GPU sharding is described in DESIGN.md)",
+              "(slices are independent: several GPUs would each take a share of the slices)",
               file=sys.stderr)
         return 2
     if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
@@ -542,6 +702,8 @@ def main() -> int:
         return run_prep(args)
     if args.workload == "ingest":
         return run_ingest(args)
+    if args.workload == "grad":
+        return run_grad(args)
 
     import numpy as np
 
