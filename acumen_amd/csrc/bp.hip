@@ -10,7 +10,7 @@
 //  * one workgroup = a 64 (x) x TY (y) pixel tile, TY = 16, 32 or 64 (a
 //    template parameter the autotuner picks per batch), one lane per x
 //    column; 4 waves (8 for TY = 64) own 8 rows each (4 for TY = 16), so
-//    one (sin, cos) pair and one LDS row serve 8 updates per lane;
+//    one (sin, cos) pair and one LDS row serve that many updates per lane;
 //  * angles are processed in blocks of 32; for each angle the tile only
 //    touches a window of W detector bins (72 / 80 / 96), which is staged in
 //    LDS as pairs P[t] = (S[lo+t], S[lo+t+1]-S[lo+t]) so an update is one
