@@ -1,0 +1,55 @@
+"""Host-side parts of bench.py and of the synthetic inputs, on the CPU: the
+one-GPU refusal, the traffic figures read from the committed counter
+summaries, and the centre-of-rotation shift of the differentiation workload
+against the oracle's shift."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+from acumen_amd import synth
+from oracle import ref
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
+
+
+def test_traffic_from_committed_profiles():
+    fpl = bench.traffic_from_profile("fpl_kernel_traffic_2049x1500.txt")
+    gather = bench.traffic_from_profile("fp_kernel_traffic_2049x1500.txt")
+    alg = 4.0 * (2049 * 2049 + 2049 * 1500)  # image + sinogram of one slice
+    assert fpl is not None and gather is not None
+    # every launch moves at least the compulsory bytes; the staged boxes move far
+    # fewer than one gather of a bilinear cell per sample does
+    assert alg < fpl < gather
+    assert bench.traffic_from_profile("no_such_profile.txt") is None
+
+
+def test_kernel_names_of_variants():
+    assert bench.fp_kernel_name("lds4") == "fpl_kernel"
+    assert bench.fp_kernel_name("lds1") == "fpl_kernel"
+    assert bench.fp_kernel_name("cell16") == "fplc_kernel"
+    assert bench.fp_kernel_name("gather") == "fp_kernel"
+
+
+def test_shift_rows_matches_oracle_shift():
+    rng = np.random.default_rng(7)
+    for T, N in ((50, 65), (720, 33)):
+        s = rng.normal(size=(T, N)).astype(np.float32)
+        sh = rng.normal(0.0, 1.5, T).astype(np.float32)
+        sh[:3] = (0.0, 2.0, -1.0)  # integer shifts are index moves
+        got = synth.shift_rows(s, sh)
+        np.testing.assert_allclose(got, ref.shift(s, sh, "f64"), rtol=0, atol=1e-6)
+        assert np.array_equal(got[0], s[0])
+        assert np.array_equal(got[1, 2:], s[1, :-2]) and np.all(got[1, :2] == 0)
+        assert np.array_equal(got[2, :-1], s[2, 1:]) and got[2, -1] == 0
