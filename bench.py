@@ -353,7 +353,7 @@ def run_prep(args) -> int:
     also checks the GPU's sinogram rows of that sample bit for bit."""
     import numpy as np
 
-    from acumen_amd import Context, prep, synth
+    from acumen_amd import Context, _lib, prep, synth
 
     n, count = N_DET, 64
     ctx = Context(0)
@@ -373,10 +373,16 @@ def run_prep(args) -> int:
     t_search = time.perf_counter() - t
     log("median + correction of", count, "projections")
     prep.sinograms(ctx, proj[:4], th, sh)  # warm-up
+    ctx.set_timing(True)
     t = time.perf_counter()
     sino = prep.sinograms(ctx, proj, th, sh)
     t_pipe = time.perf_counter() - t
+    k_ms, _ = ctx.kernel_time(_lib.KIND_PREP)
+    ctx.set_timing(False)
     ctx.close()
+    # compulsory HBM bytes per projection: read it, write its median, read the
+    # median, write the corrected sinogram rows
+    prep_bytes = 4 * 4.0 * n * n * count
     cpu, exact = None, None
     if not args.no_cpu:
         log("cpu baseline: the oracle's median + correction of 4 projections")
@@ -396,6 +402,9 @@ def run_prep(args) -> int:
         "data": "synthetic",
         "config": {"workload": f"{count} projections of {n}^2, axis 3.4 px off centre", "n": n,
                    "projections": count},
+        "kernels_ms": round(k_ms, 3),
+        "kernels_projections_per_s": round(count / (k_ms * 1e-3), 1),
+        "kernels_compulsory_hbm_GBps": round(prep_bytes / (k_ms * 1e-3) / 1e9, 1),
         "center_search_s": round(t_search, 3),
         "center_search_evaluations": evals,
         "found_shift_px": round(sh * n, 4),
