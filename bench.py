@@ -347,8 +347,9 @@ def prep_cpu_baseline(proj, sino, th, sh, ks):
 def run_prep(args) -> int:
     """Projection -> sinogram preprocessing at 2048^2 projections: the
     centre-of-rotation / tilt search on the 0 and 180 degree projections,
-    and the median filter + correction of 64 projections into sinogram rows
-    (host -> GPU -> host, PCIe included).  cpu_baseline: the oracle's
+    and the median filter + correction of 64 projections into sinogram rows.
+    value: the kernels' rate (their HIP-event time); the host -> GPU -> host
+    pipeline rate, PCIe and pageable copies included, is reported beside it.  cpu_baseline: the oracle's
     median + correction of a sample of projections on one host thread, which
     also checks the GPU's sinogram rows of that sample bit for bit."""
     import numpy as np
@@ -388,13 +389,14 @@ def run_prep(args) -> int:
         log("cpu baseline: the oracle's median + correction of 4 projections")
         cpu, exact = prep_cpu_baseline(proj, sino, th, sh, [1, count // 3, count // 2, count - 2])
     out = {
-        "metric": "projections/s through median filter + correction into sinogram rows (PCIe incl.), 2048^2",
-        "value": round(count / t_pipe, 2),
+        "metric": "projections/s through median filter + correction into sinogram rows, 2048^2 "
+                  "(kernels on resident data; the host-to-host pipeline rate beside it)",
+        "value": round(count / (k_ms * 1e-3), 1),
         "unit": "projections/s",
         "n_gpus": 1,
         "steps": 1,
         "warmup": 1,
-        "ms_per_step": round(t_pipe * 1e3, 2),
+        "ms_per_step": round(k_ms, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -402,8 +404,8 @@ def run_prep(args) -> int:
         "data": "synthetic",
         "config": {"workload": f"{count} projections of {n}^2, axis 3.4 px off centre", "n": n,
                    "projections": count},
-        "kernels_ms": round(k_ms, 3),
-        "kernels_projections_per_s": round(count / (k_ms * 1e-3), 1),
+        "pipeline_projections_per_s_pcie": round(count / t_pipe, 2),
+        "pipeline_ms": round(t_pipe * 1e3, 2),
         "kernels_compulsory_hbm_GBps": round(prep_bytes / (k_ms * 1e-3) / 1e9, 1),
         "center_search_s": round(t_search, 3),
         "center_search_evaluations": evals,
