@@ -9,6 +9,7 @@
 // corrected projection row jy of projection `job` is sinogram row `job` of
 // slice jy, Nw = n + add_Nx values long.
 #include "acu_prep.hpp"
+#include "median_net.hpp"
 
 #include <algorithm>
 
@@ -20,26 +21,23 @@ constexpr int MT = 16;  // median tile side (16 x 16 threads)
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
 
-// Ascending bitonic sort of 32 values in registers (fully unrolled).
-__device__ __forceinline__ void bitonic32(float (&v)[32]) {
+// The 14th smallest of 27 values in registers: the compile-time network of
+// median_net.hpp, fully unrolled (153 compare-exchanges).
+__device__ __forceinline__ float median27(float (&v)[27]) {
 #pragma unroll
-  for (int k = 2; k <= 32; k <<= 1)
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1)
-#pragma unroll
-      for (int i = 0; i < 32; ++i) {
-        const int l = i ^ j;
-        if (l > i) {
-          const float a = v[i], b = v[l];
-          const bool up = (i & k) == 0;
-          v[i] = up ? fminf(a, b) : fmaxf(a, b);
-          v[l] = up ? fmaxf(a, b) : fminf(a, b);
-        }
-      }
+  for (int i = 0; i < MEDIAN27_NET.n; ++i) {
+    const float a = v[MEDIAN27_NET.lo[i]], b = v[MEDIAN27_NET.hi[i]];
+    v[MEDIAN27_NET.lo[i]] = fminf(a, b);
+    v[MEDIAN27_NET.hi[i]] = fmaxf(a, b);
+  }
+  return v[MEDIAN27_NET.out];
 }
 
 // 3x3x3 median (:420-447): projections l, c, r; neighbours clamped into the
 // image; the 14th of 27 in descending order = the 14th in ascending order.
+// Values that compare equal but differ in bits (+0 and -0) or NaN are the
+// exception: the reference's exchange sort is not stable, so which of them
+// it leaves in the middle depends on its order of exchanges.
 __global__ __launch_bounds__(MT * MT) void median27_kernel(int n, const float* __restrict__ l,
                                                            const float* __restrict__ c,
                                                            const float* __restrict__ r, float* __restrict__ out,
@@ -58,17 +56,14 @@ __global__ __launch_bounds__(MT * MT) void median27_kernel(int n, const float* _
   const int tx = threadIdx.x % MT, ty = threadIdx.x / MT;
   const int x = x0 + tx, y = y0 + ty;
   if (x >= n || y >= n) return;  // after the only barrier
-  float v[32];
+  float v[27];
 #pragma unroll
   for (int p = 0; p < 3; ++p)
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
       for (int dx = 0; dx < 3; ++dx) v[p * 9 + dy * 3 + dx] = t[p][ty + dy][tx + dx];
-#pragma unroll
-  for (int i = 27; i < 32; ++i) v[i] = __builtin_inff();
-  bitonic32(v);
-  out[(size_t)y * n + x] = v[13];
+  out[(size_t)y * n + x] = median27(v);
 }
 
 // The resampling of :449-464 at output (jx, jy), 1-based, in the
