@@ -42,15 +42,27 @@ __global__ __launch_bounds__(MT * MT) void median27_kernel(int n, const float* _
                                                            const float* __restrict__ c,
                                                            const float* __restrict__ r, float* __restrict__ out,
                                                            int tiles_x) {
-  __shared__ float t[3][MT + 2][MT + 2];
+  constexpr int H = MT + 2, HH = H * H, NS = (3 * HH + MT * MT - 1) / (MT * MT);
+  __shared__ float t[3 * HH];  // [p][yy][xx]
   const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
   const int x0 = bx * MT, y0 = by * MT;  // 0-based tile origin
-  const float* src[3] = {l, c, r};
-  for (int i = threadIdx.x; i < 3 * (MT + 2) * (MT + 2); i += MT * MT) {
-    const int p = i / ((MT + 2) * (MT + 2)), q = i - p * (MT + 2) * (MT + 2);
-    const int yy = q / (MT + 2), xx = q - yy * (MT + 2);
+  // halo tiles of the three projections: every load issued before the first
+  // LDS store (element i = p*HH + yy*H + xx; the divisions are by constants)
+  float st[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const int i = (int)threadIdx.x + k * MT * MT;
+    const int ic = i < 3 * HH ? i : 0;
+    const int p = ic / HH, q = ic - p * HH;
+    const int yy = q / H, xx = q - yy * H;
     const int gx = clampi(x0 + xx - 1, 0, n - 1), gy = clampi(y0 + yy - 1, 0, n - 1);
-    t[p][yy][xx] = src[p][(size_t)gy * n + gx];
+    const float* sp = p == 0 ? l : p == 1 ? c : r;  // no private-memory pointer table
+    st[k] = sp[(size_t)gy * n + gx];
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const int i = (int)threadIdx.x + k * MT * MT;
+    if (i < 3 * HH) t[i] = st[k];
   }
   __syncthreads();
   const int tx = threadIdx.x % MT, ty = threadIdx.x / MT;
@@ -62,7 +74,7 @@ __global__ __launch_bounds__(MT * MT) void median27_kernel(int n, const float* _
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-      for (int dx = 0; dx < 3; ++dx) v[p * 9 + dy * 3 + dx] = t[p][ty + dy][tx + dx];
+      for (int dx = 0; dx < 3; ++dx) v[p * 9 + dy * 3 + dx] = t[p * HH + (ty + dy) * H + tx + dx];
   out[(size_t)y * n + x] = median27(v);
 }
 
