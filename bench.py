@@ -16,11 +16,13 @@ roofline: the forward projector dominates TV time (6 of the 13 projector
 launches per outer iteration and ~74% of the time); its algorithmic work is
 14 FP32 flops per ray sample (2 coordinates, 2 fractions, 3 lerps, 1
 accumulate) over N*N*T samples per slice, measured with HIP events on the
-library's stream during the timed steps.  The kernel is co-limited by the
-VALU issue rate and its LDS gathers (16 bytes of staged box per sample), so
-`roofline.lds` reports that rate as well.  `roofline_backprojection` gives
-the same figures for the backprojector (6 flop per pixel-angle update over
-N*N*T updates, compulsory bytes 4(N*T + 2*N*N) per slice).
+library's stream during the timed steps.  The kernel is bound by its LDS
+gathers (16 bytes of staged box per sample), so `roofline.lds` reports that
+rate as well.  `roofline_backprojection` gives the same figures for the
+backprojector (6 flop per pixel-angle update over N*N*T updates, compulsory
+bytes 4(N*T + 2*N*N) per slice) and its LDS read rate: 8 bytes per update
+over the updates of the tiles that hold mask pixels (the others only store
+zeros).
 
 cpu_baseline: the oracle's FP32 restatement of the reference's loops, one
 slice per host thread as the reference's MPI ranks do, on a bounded sample
@@ -90,6 +92,23 @@ def traffic_from_profile(name=None, key="hbm_bytes_per_launch_per_slice"):
         if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def bp_active_fraction(N, wpu, ty):
+    """Fraction of the N*N pixels the backprojector computes: its 64 x ty
+    tiles that hold a pixel of nonzero mask (tomography/tv_hpc.F90:58-74,
+    nonzero where 1 - 2(r - rad_n)/wpu > 0)."""
+    import numpy as np
+    half = (np.float32(N) - 1) / 2
+    d = (np.arange(1, N + 1, dtype=np.float32) - half) - 1
+    r = np.sqrt(d[None, :] ** 2 + d[:, None] ** 2)
+    rad_n = np.float32((N - 1) / 2) - np.float32(wpu)
+    on = (1 - (2 * (r - rad_n)) / np.float32(wpu)) > 0
+    nty, ntx = -(-N // ty), -(-N // 64)
+    pad = np.zeros((nty * ty, ntx * 64), bool)
+    pad[:N, :N] = on
+    tiles = pad.reshape(nty, ty, ntx, 64).any(axis=(1, 3))
+    return float(tiles.sum()) * 64 * ty / (N * N)
 
 
 def cpu_baseline(N, T, wpu, threads):
@@ -821,8 +840,11 @@ def main() -> int:
     bp_ms, bp_n = kt["bp"]
     bp_avg_s = bp_ms / max(bp_n, 1) * 1e-3
     bp_achieved = BP_FLOPS_PER_UPDATE * samples_per_launch / bp_avg_s / 1e12
+    bp_ty = tuning["bp_tile"] if tuning else 32
+    bp_active = bp_active_fraction(N, wpu, bp_ty)
+    bp_lds = 8.0 * samples_per_launch * bp_active / bp_avg_s
     roofline_bp = {
-        "bound": "valu",
+        "bound": "lds",
         "kernel": "bp_kernel (pixel-driven linear-interpolation backprojection on the FP32 VALU)",
         "achieved": round(bp_achieved, 3),
         "peak": FP32_PEAK_TFLOPS,
@@ -832,6 +854,10 @@ def main() -> int:
         "launches": bp_n,
         "work_per_launch": f"{BP_FLOPS_PER_UPDATE} flop x {N}*{N}*{T} updates x {B} slices",
         "compulsory_hbm_GBps": round(4.0 * (N * T + 2.0 * N * N) * B / bp_avg_s / 1e9, 1),
+        # 8 bytes of staged sinogram pairs per update, over the tiles that hold mask pixels
+        "lds": {"bytes_per_update": 8, "tile_rows": bp_ty, "computed_fraction": round(bp_active, 4),
+                "achieved_TBps": round(bp_lds / 1e12, 2),
+                "B_per_clk_per_cu": round(bp_lds / (N_CU * CLOCK_GHZ * 1e9), 1)},
     }
 
     cpu = None

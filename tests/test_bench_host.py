@@ -35,6 +35,16 @@ def test_traffic_from_committed_profiles():
     assert bench.traffic_from_profile("no_such_profile.txt") is None
 
 
+def test_backprojection_active_fraction_matches_oracle_mask():
+    """The share of pixels the backprojector computes: 64 x ty tiles that hold
+    a pixel where the oracle's mask is nonzero."""
+    for N, wpu, ty in ((65, 4, 16), (257, 8, 32), (129, 16, 64)):
+        on = ref.cmask(N, 180, wpu) != 0
+        nty, ntx = -(-N // ty), -(-N // 64)
+        tiles = sum(on[y * ty:(y + 1) * ty, x * 64:(x + 1) * 64].any() for y in range(nty) for x in range(ntx))
+        assert bench.bp_active_fraction(N, wpu, ty) == tiles * 64 * ty / (N * N)
+
+
 def test_kernel_names_of_variants():
     assert bench.fp_kernel_name("lds4") == "fpl_kernel"
     assert bench.fp_kernel_name("lds1") == "fpl_kernel"
