@@ -844,7 +844,7 @@ def main() -> int:
     bp_active = bp_active_fraction(N, wpu, bp_ty)
     bp_lds = 8.0 * samples_per_launch * bp_active / bp_avg_s
     roofline_bp = {
-        "bound": "lds",
+        "bound": "lds+valu",
         "kernel": "bp_kernel (pixel-driven linear-interpolation backprojection on the FP32 VALU)",
         "achieved": round(bp_achieved, 3),
         "peak": FP32_PEAK_TFLOPS,
