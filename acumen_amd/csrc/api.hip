@@ -64,6 +64,26 @@ struct acu_ctx {
     ACU_HIP_CHECK(hipStreamSynchronize(stream));
     drain();
   }
+  // Two pinned host buffers for calls that take pageable host arrays: a host
+  // copy into one overlaps the DMA of the other.  Grown on demand (after a
+  // stream sync, since a DMA may still use the old ones) and held until the
+  // context is destroyed.
+  float* pin[2] = {nullptr, nullptr};
+  size_t pin_bytes = 0;
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};  // last DMA through each buffer
+  void pinned(size_t bytes) {
+    if (!pin_ev[0])
+      for (auto& e : pin_ev) ACU_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (bytes <= pin_bytes) return;
+    sync();
+    for (auto& b : pin) {
+      if (b) ACU_HIP_CHECK(hipHostFree(b));
+      b = nullptr;
+    }
+    pin_bytes = 0;
+    for (auto& b : pin) ACU_HIP_CHECK(hipHostMalloc((void**)&b, bytes, hipHostMallocDefault));
+    pin_bytes = bytes;
+  }
 };
 
 namespace acu {
@@ -1163,6 +1183,10 @@ int acu_ctx_destroy(acu_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   for (auto& p : ctx->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
   for (auto e : ctx->pool) hipEventDestroy(e);
+  for (auto e : ctx->pin_ev)
+    if (e) hipEventDestroy(e);
+  for (auto b : ctx->pin)
+    if (b) hipHostFree(b);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return ACU_OK;
@@ -1619,6 +1643,50 @@ void par_copy(void* dst, const void* src, size_t bytes) {
   for (auto& x : th) x.join();
 }
 
+// Pageable host -> device through the context's pinned buffers: the host
+// copy of this piece overlaps the DMA of the previous one (slot alternates).
+void stage_h2d(acu_ctx* c, void* dst_d, const void* src_h, size_t bytes, int& slot) {
+  c->pinned(bytes);
+  ACU_HIP_CHECK(hipEventSynchronize(c->pin_ev[slot]));  // the buffer's previous DMA is done
+  par_copy(c->pin[slot], src_h, bytes);
+  ACU_HIP_CHECK(hipMemcpyAsync(dst_d, c->pin[slot], bytes, hipMemcpyHostToDevice, c->stream));
+  ACU_HIP_CHECK(hipEventRecord(c->pin_ev[slot], c->stream));
+  slot ^= 1;
+}
+
+// A pageable host array -> device in pieces of up to 32 MiB through the
+// pinned buffers (stage_h2d); the DMAs are queued on the context's stream.
+void stage_h2d_pieces(acu_ctx* c, void* dst_d, const void* src_h, size_t bytes) {
+  const size_t piece = (size_t)32 << 20;
+  int slot = 0;
+  for (size_t off = 0; off < bytes; off += piece)
+    stage_h2d(c, (char*)dst_d + off, (const char*)src_h + off, std::min(piece, bytes - off), slot);
+}
+
+// Device -> pageable host in pieces through the pinned buffers: the DMA of
+// one piece overlaps the host copy of the previous one.  Returns when every
+// byte is on the host.
+void stage_d2h(acu_ctx* c, void* dst_h, const void* src_d, size_t bytes) {
+  const size_t piece = std::min(bytes, (size_t)32 << 20);
+  if (piece == 0) return;
+  c->pinned(piece);
+  size_t prev_off = 0, prev_len = 0;
+  int prev = -1;
+  for (size_t off = 0, k = 0; off < bytes; off += piece, ++k) {
+    const size_t len = std::min(piece, bytes - off);
+    const int slot = (int)(k & 1);  // its previous piece was copied out in the last iteration
+    ACU_HIP_CHECK(hipMemcpyAsync(c->pin[slot], (const char*)src_d + off, len, hipMemcpyDeviceToHost, c->stream));
+    ACU_HIP_CHECK(hipEventRecord(c->pin_ev[slot], c->stream));
+    if (prev >= 0) {
+      ACU_HIP_CHECK(hipEventSynchronize(c->pin_ev[prev]));
+      par_copy((char*)dst_h + prev_off, c->pin[prev], prev_len);
+    }
+    prev = slot; prev_off = off; prev_len = len;
+  }
+  ACU_HIP_CHECK(hipEventSynchronize(c->pin_ev[prev]));
+  par_copy((char*)dst_h + prev_off, c->pin[prev], prev_len);
+}
+
 acu_batch* make_volume_batch(acu_ctx* c, int n, int N, int T, int wpu, const float* theta, const acu_batch* like) {
   std::vector<int> Ns(n, N), Ts(n, T), ws(n, wpu);
   std::vector<const float*> th(n, theta);
@@ -1829,10 +1897,10 @@ void prep_sinograms(acu_ctx* c, int n, int nfiles, const float* proj, float thet
   for (int j0 = 0; j0 < nfiles; j0 += B) {
     const int cnt = std::min(B, nfiles - j0);
     // window slot w holds projection j0 - 1 + w (clamped to the range)
+    int slot = 0;
     for (int w = 0; w < cnt + 2; ++w) {
       const int j = std::min(std::max(j0 - 1 + w, 0), nfiles - 1);
-      ACU_HIP_CHECK(hipMemcpyAsync(win + (size_t)w * nn, proj + (size_t)j * nn, nn * 4, hipMemcpyHostToDevice,
-                                   c->stream));
+      stage_h2d(c, win + (size_t)w * nn, proj + (size_t)j * nn, nn * 4, slot);
     }
     acu::timed(c, acu::K_PREP, [&] {
       for (int k = 0; k < cnt; ++k)
@@ -1842,7 +1910,7 @@ void prep_sinograms(acu_ctx* c, int n, int nfiles, const float* proj, float thet
     });
     ACU_HIP_CHECK(hipGetLastError());
   }
-  ACU_HIP_CHECK(hipMemcpyAsync(sino_h, sino, (size_t)n * nfiles * nw * 4, hipMemcpyDeviceToHost, c->stream));
+  stage_d2h(c, sino_h, sino, (size_t)n * nfiles * nw * 4);
   c->sync();
 }
 }  // namespace
@@ -1904,9 +1972,9 @@ int acu_ingest_mean(acu_ctx* ctx, int count, long long npix, const float* imgs, 
   ACU_TRY(hipSetDevice(ctx->device);
           const size_t np = (size_t)npix;
           DevBuf in(np * count), o(np);
-          ACU_HIP_CHECK(hipMemcpyAsync(in.p, imgs, np * count * 4, hipMemcpyHostToDevice, ctx->stream));
+          stage_h2d_pieces(ctx, in.p, imgs, np * count * 4);
           acu::timed(ctx, acu::K_INGEST, [&] { acu::launch_mean(in.p, count, np, o.p, ctx->stream); }); ACU_HIP_CHECK(hipGetLastError());
-          ACU_HIP_CHECK(hipMemcpyAsync(out, o.p, np * 4, hipMemcpyDeviceToHost, ctx->stream));
+          stage_d2h(ctx, out, o.p, np * 4);
           ctx->sync());
 }
 
@@ -1917,10 +1985,10 @@ int acu_ingest_normalize(acu_ctx* ctx, int count, long long npix, const float* i
   ACU_TRY(hipSetDevice(ctx->device);
           const size_t np = (size_t)npix;
           DevBuf in(np * count), ob(np), o(np * count);
-          ACU_HIP_CHECK(hipMemcpyAsync(in.p, imgs, np * count * 4, hipMemcpyHostToDevice, ctx->stream));
-          ACU_HIP_CHECK(hipMemcpyAsync(ob.p, open_beam, np * 4, hipMemcpyHostToDevice, ctx->stream));
+          stage_h2d_pieces(ctx, in.p, imgs, np * count * 4);
+          stage_h2d_pieces(ctx, ob.p, open_beam, np * 4);
           acu::timed(ctx, acu::K_INGEST, [&] { acu::launch_normalize(in.p, count, np, ob.p, o.p, ctx->stream); }); ACU_HIP_CHECK(hipGetLastError());
-          ACU_HIP_CHECK(hipMemcpyAsync(out, o.p, np * count * 4, hipMemcpyDeviceToHost, ctx->stream));
+          stage_d2h(ctx, out, o.p, np * count * 4);
           ctx->sync());
 }
 

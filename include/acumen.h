@@ -63,7 +63,10 @@ typedef struct acu_batch acu_batch;
 /* Last error message of the calling thread (never NULL). */
 const char *acu_last_error(void);
 
-/* Context: device selection and the stream all work is ordered on. */
+/* Context: device selection and the stream all work is ordered on.  Calls
+ * that take pageable host arrays of projections (acu_prep_sinograms,
+ * acu_ingest_*) stage them through two pinned host buffers that the context
+ * allocates on first use (up to 32 MiB each) and frees in acu_ctx_destroy. */
 int acu_ctx_create(int device, acu_ctx **out);
 int acu_ctx_destroy(acu_ctx *ctx);
 int acu_ctx_synchronize(acu_ctx *ctx);
