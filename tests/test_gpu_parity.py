@@ -331,6 +331,28 @@ def test_tiny_and_degenerate_shapes(ctx, N, T, wpu):
     assert rel(img, ref.iradon(s, th, wpu, "f64")) < 1e-5
 
 
+@pytest.mark.parametrize("N,T", [(64, 48), (256, 180), (130, 37)])
+def test_even_image_side(ctx, N, T):
+    """Even N (the drivers pad detectors to odd widths, but the module takes
+    any N): the midpoint (N+1)/2 is then N/2 by integer division, off the
+    image centre.  radon, FBP and TV against the oracle as for odd N."""
+    say("even N", N, T)
+    th = synth.angles(T) if T != 37 else np.sort(np.random.default_rng(T).uniform(0, 180, T)).astype(np.float32)
+    wpu = synth.default_wpu(N)
+    x = disk_phantom(N, 6, N + T)
+    with plan(ctx, N, T, th) as b:
+        s = b.radon(x)
+        img = b.iradon(s)
+        b.set_sino(0, s)
+        b.tvopt(0.05, 0.1, 2, 1, 4, ACU_TV_GRAPH)
+        tv = b.get_image(0, ACU_IMG_TV)
+    assert rel(s, ref.radon(x, th, "f64")) < FBP_TOL
+    assert rel(s, ref.radon(x, th, "f32")) < FBP_TOL
+    assert rel(img, ref.iradon(s, th, wpu, "f64")) < FBP_TOL
+    tv64, _, _ = ref.tvopt(s, th, wpu, 0.05, 0.1, 2, 1, 4, "f64")
+    assert rel(tv, tv64) < TV_TOL
+
+
 def test_rejects_bad_arguments(ctx):
     from acumen_amd import AcumenError
     with pytest.raises(AcumenError):
