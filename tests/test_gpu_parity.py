@@ -353,6 +353,30 @@ def test_even_image_side(ctx, N, T):
     assert rel(tv, tv64) < TV_TOL
 
 
+@pytest.mark.parametrize("N,T", [(129, 97), (257, 400)])
+def test_unsorted_repeated_and_out_of_range_angles(ctx, N, T):
+    """The module takes any angle list: unsorted, with repeats, below 0 and
+    above 180 degrees (the projector's angle blocks and box sizes follow the
+    table as given)."""
+    say("angle set", N, T)
+    rng = np.random.default_rng(N * T)
+    th = rng.uniform(-60.0, 240.0, T).astype(np.float32)
+    th[T // 3:T // 3 + 5] = th[0]          # repeats
+    th[-3:] = (-90.0, 270.0, 360.0)          # special angles beyond the usual range
+    wpu = synth.default_wpu(N)
+    x = disk_phantom(N, 6, N + 5)
+    with plan(ctx, N, T, th) as b:
+        s = b.radon(x)
+        img = b.iradon(s)
+        b.set_sino(0, s)
+        b.tvopt(0.05, 0.1, 1, 1, 3, ACU_TV_GRAPH)
+        tv = b.get_image(0, ACU_IMG_TV)
+    assert rel(s, ref.radon(x, th, "f64")) < FBP_TOL
+    assert rel(img, ref.iradon(s, th, wpu, "f64")) < FBP_TOL
+    tv64, _, _ = ref.tvopt(s, th, wpu, 0.05, 0.1, 1, 1, 3, "f64")
+    assert rel(tv, tv64) < TV_TOL
+
+
 def test_rejects_bad_arguments(ctx):
     from acumen_amd import AcumenError
     with pytest.raises(AcumenError):
