@@ -377,6 +377,38 @@ def test_unsorted_repeated_and_out_of_range_angles(ctx, N, T):
     assert rel(tv, tv64) < TV_TOL
 
 
+@pytest.mark.parametrize("N,T", [(4097, 24), (8193, 8)])
+def test_large_images(ctx, N, T):
+    """Image sides beyond the production size: 4097 (the largest the FFT
+    filter takes) and 8193 (Toeplitz filter only), radon and FBP against the
+    oracle, with the library's defaults and with the FFT filter where it
+    applies.  The reference's FP32 taps and sums put its own FBP 5.5e-5
+    (4097) and 7.7e-5 (8193) from FP64; the GPU's must be no further."""
+    say("large", N, T)
+    th = synth.angles(T)
+    wpu = synth.default_wpu(N)
+    x = disk_phantom(N, 12, N)
+    with plan(ctx, N, T, th) as b:
+        s = b.radon(x)
+        img = b.iradon(s)
+        fft = None
+        if N <= 4097:
+            b.set_ramp_variant(_lib.RAMP_FFT)
+            fft = b.iradon(s)
+    ref.set_threads(16)
+    try:
+        s64 = ref.radon(x, th, "f64")
+        img64 = ref.iradon(s, th, wpu, "f64")
+        bar = max(FBP_TOL, rel(ref.iradon(s, th, wpu, "f32"), img64))
+    finally:
+        ref.set_threads(1)
+    assert rel(s, s64) < FBP_TOL
+    say("FBP vs FP64", rel(img, img64), "reference FP32 vs FP64", bar)
+    assert rel(img, img64) <= bar
+    if fft is not None:
+        assert rel(fft, img64) <= bar, rel(fft, img64)
+
+
 def test_rejects_bad_arguments(ctx):
     from acumen_amd import AcumenError
     with pytest.raises(AcumenError):
