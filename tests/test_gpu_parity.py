@@ -409,6 +409,61 @@ def test_large_images(ctx, N, T):
         assert rel(fft, img64) <= bar, rel(fft, img64)
 
 
+def test_extreme_mixed_batch_equals_single_plans(ctx):
+    """Slices of N = 3, 65 and 2049 (1, 1000 and 40 angles) in one batch:
+    graph-replayed TV equals the single-slice plans bit for bit."""
+    shapes = [(3, 1, 1), (2049, 40, synth.default_wpu(2049)), (65, 1000, synth.default_wpu(65))]
+    say("extreme mix", shapes)
+    items = []
+    for i, (N, T, w) in enumerate(shapes):
+        th = synth.angles(T)
+        f = ref.radon(disk_phantom(N, 4, 70 + i) if N > 8 else np.ones((N, N), np.float32), th, "f64")
+        items.append((N, T, w, th, f.astype(np.float32)))
+    with Batch(ctx, [(N, T, w, th) for N, T, w, th, _ in items]) as b:
+        for i, it in enumerate(items):
+            b.set_sino(i, it[4])
+        b.tvopt(0.05, 0.1, 2, 1, 3, ACU_TV_GRAPH)
+        batched = [b.get_image(i, ACU_IMG_TV) for i in range(len(items))]
+    for i, (N, T, w, th, f) in enumerate(items):
+        with plan(ctx, N, T, th, w) as p:
+            p.set_sino(0, f)
+            p.tvopt(0.05, 0.1, 2, 1, 3, ACU_TV_PLAIN)
+            assert np.array_equal(p.get_image(0, ACU_IMG_TV), batched[i]), N
+
+
+def test_contexts_on_two_threads_equal_sequential_runs():
+    """One context per host thread: two threads running batches at once give
+    the bits the same calls give one after another."""
+    import threading
+    from acumen_amd import Context
+    cases = [(257, 180, 11), (129, 360, 12)]
+    sinos = {(N, T, seed): [ref.radon(disk_phantom(N, 5, seed + i), synth.angles(T), "f64").astype(np.float32)
+                            for i in range(2)] for N, T, seed in cases}
+
+    def run(N, T, seed, out, k):
+        th = synth.angles(T)
+        with Context(0) as c, Batch(c, [(N, T, synth.default_wpu(N), th)] * 2) as b:
+            for i in range(2):
+                b.set_sino(i, sinos[(N, T, seed)][i])
+            b.tvopt(0.05, 0.1, 2, 1, 3, ACU_TV_GRAPH)
+            out[k] = [b.get_image(i, ACU_IMG_TV) for i in range(2)] + [b.get_image(i, ACU_IMG_FBP) for i in range(2)]
+
+    say("two threads")
+    seq = [None, None]
+    for k, cse in enumerate(cases):
+        run(*cse, seq, k)
+    par = [None, None]
+    threads = [threading.Thread(target=run, args=(*cse, par, k)) for k, cse in enumerate(cases)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert all(p is not None for p in par), "a thread did not finish"
+    for a, b in zip(seq, par):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
 def test_rejects_bad_arguments(ctx):
     from acumen_amd import AcumenError
     with pytest.raises(AcumenError):
