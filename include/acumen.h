@@ -156,7 +156,11 @@ int acu_tvopt(acu_batch *b, const float *f_host, float mu, float lambda, int nou
  * whole batch and keeps the fastest, and the backprojection tile height
  * (below) on the whole batch.  report (may be NULL) receives a JSON summary
  * of the timings and choices, truncated to cap bytes including the
- * terminating NUL.  The
+ * terminating NUL.  The choices follow measured times, so two runs may
+ * choose differently; projector variants 4 and 8 agree bit for bit where
+ * their boxes fit, while the ramp arithmetics and the tile heights differ
+ * at FP32 rounding level.  A caller that needs identical bits from run to
+ * run stores the choices and reapplies them with the setters.  The
  * getter returns the variant, or -ACU_ERR_ARG for a bad slice; the setter
  * lets a caller reapply a stored choice. */
 int acu_batch_autotune(acu_batch *b, int reps, char *report, int cap);
