@@ -103,16 +103,14 @@ __device__ __forceinline__ float prep_sample(const PrepGeom& g, const float* __r
 
 // Corrected projections of a batch -> sinogram rows: projection k of the
 // batch (global index job0 + k) fills row job0 + k of every slice's
-// sinogram; sino is nslices(= n) x nfiles x Nw, slice-major.
-__global__ __launch_bounds__(256) void correct_kernel(PrepGeom g, const float* __restrict__ imgs, int count,
-                                                      int job0, int nfiles, float* __restrict__ sino) {
+// sinogram; sino is nslices(= n) x nfiles x Nw, slice-major.  Grid: x over
+// the Nw output columns, y = the detector row jy - 1 - row0, z = the projection.
+__global__ __launch_bounds__(256) void correct_kernel(PrepGeom g, const float* __restrict__ imgs, int job0,
+                                                      int nfiles, int row0, float* __restrict__ sino) {
   const int nw = g.n + g.add_nx;
-  const size_t per = (size_t)nw * g.n;
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= per * count) return;
-  const int k = (int)(i / per);
-  const size_t r = i - (size_t)k * per;
-  const int jy = (int)(r / nw) + 1, jx = (int)(r - (size_t)(jy - 1) * nw) + 1;
+  const int jx = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;
+  if (jx > nw) return;
+  const int jy = row0 + (int)blockIdx.y + 1, k = (int)blockIdx.z;
   const float v = prep_sample(g, imgs + (size_t)k * g.n * g.n, jx, jy);
   sino[((size_t)(jy - 1) * nfiles + (size_t)(job0 + k)) * nw + (jx - 1)] = v;
 }
@@ -156,9 +154,13 @@ void launch_median27(int n, const float* l, const float* c, const float* r, floa
 
 void launch_correct(const PrepGeom& g, const float* imgs, int count, int job0, int nfiles, float* sino,
                     hipStream_t s) {
-  const size_t total = (size_t)(g.n + g.add_nx) * g.n * count;
-  hipLaunchKernelGGL(correct_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, imgs, count, job0,
-                     nfiles, sino);
+  const unsigned gx = (unsigned)((g.n + g.add_nx + 255) / 256);
+  for (int k0 = 0; k0 < count; k0 += 65535)  // the grid's y and z extents are at most 65535
+    for (int r0 = 0; r0 < g.n; r0 += 65535) {
+      const int cnt = std::min(65535, count - k0), rows = std::min(65535, g.n - r0);
+      hipLaunchKernelGGL(correct_kernel, dim3(gx, (unsigned)rows, (unsigned)cnt), dim3(256), 0, s, g,
+                         imgs + (size_t)k0 * g.n * g.n, job0 + k0, nfiles, r0, sino);
+    }
 }
 
 int mirror_blocks(const PrepGeom& g) { return (int)(((size_t)(g.n + g.add_nx) * g.n + 255) / 256); }
