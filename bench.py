@@ -6,9 +6,9 @@ angles on [0, 180), a 2048^2 phantom of 40 seeded ellipses, Poisson noise at
 I0 = 5e3, Ram-Lak FBP warm start then split-Bregman TV with mu = 0.05,
 lambda = 0.1, 50 outer x 1 inner x 5 CG iterations -- FP32 operators, FP64
 CG scalars.  A step is that reconstruction for a batch of `--batch` slices
-resident in HBM, run by the batched path (every kernel launch covers all
-slices, CG scalars stay on the device).  `value` is slices/s over the timed
-steps.  Alongside it: the plain per-slice GPU path (one slice at a time, each
+(default 32, one timed step of ~36 s) resident in HBM, run by the batched
+path (every kernel launch covers all slices, CG scalars stay on the device).
+`value` is slices/s over the timed steps.  Alongside it: the plain per-slice GPU path (one slice at a time, each
 reduction returned to the host before the next kernel, as the reference is
 structured) and FBP slices/s of the batched path.
 
@@ -693,9 +693,10 @@ def run_grad(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 2; ingest: 50)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: production 1 step of 32 slices; ingest 50; others 2)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16, help="slices per step (production workload)")
+    ap.add_argument("--batch", type=int, default=32, help="slices per step (production workload)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-slice leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -714,7 +715,7 @@ def main() -> int:
                     help="volume workload: slices streamed through FBP + 30-iteration TV")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = 50 if args.workload == "ingest" else 2
+        args.steps = {"ingest": 50, "production": 1}.get(args.workload, 2)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(slices are independent: several GPUs would each take a share of the slices)",
