@@ -1388,7 +1388,8 @@ int acu_batch_get_delta(acu_batch* b, int slice, float* delta_host, int n) {
     return fail(ACU_ERR_ARG, "acu_batch_get_delta: bad argument");
   auto& s = b->s[slice];
   if (!s.delta || n > s.delta_cap) return fail(ACU_ERR_STATE, "no delta log of that length");
-  ACU_TRY(ACU_HIP_CHECK(hipMemcpyAsync(delta_host, s.delta, (size_t)n * 4, hipMemcpyDeviceToHost,
+  ACU_TRY(hipSetDevice(b->ctx->device);
+          ACU_HIP_CHECK(hipMemcpyAsync(delta_host, s.delta, (size_t)n * 4, hipMemcpyDeviceToHost,
                                        b->ctx->stream));
           b->ctx->sync());
 }
@@ -1430,10 +1431,12 @@ int acu_radon(acu_batch* b, float* sino_host, const float* img_host) {
   ACU_TRY(hipSetDevice(b->ctx->device);
           acu::Slice& s = acu::slice_of(b, 0);
           DevBuf img((size_t)s.N * s.N), sino((size_t)s.N * s.T);
-          ACU_HIP_CHECK(hipMemcpy(img.p, img_host, (size_t)s.N * s.N * 4, hipMemcpyHostToDevice));
+          ACU_HIP_CHECK(hipMemcpyAsync(img.p, img_host, (size_t)s.N * s.N * 4, hipMemcpyHostToDevice,
+                                       b->ctx->stream));
           acu::radon_dev(b, 0, sino.p, img.p);
-          b->ctx->sync();
-          ACU_HIP_CHECK(hipMemcpy(sino_host, sino.p, (size_t)s.N * s.T * 4, hipMemcpyDeviceToHost)));
+          ACU_HIP_CHECK(hipMemcpyAsync(sino_host, sino.p, (size_t)s.N * s.T * 4, hipMemcpyDeviceToHost,
+                                       b->ctx->stream));
+          b->ctx->sync());
 }
 
 int acu_iradon(acu_batch* b, float* img_host, const float* sino_host) {
@@ -1441,11 +1444,13 @@ int acu_iradon(acu_batch* b, float* img_host, const float* sino_host) {
   ACU_TRY(hipSetDevice(b->ctx->device);
           acu::Slice& s = acu::slice_of(b, 0);
           DevBuf img((size_t)s.N * s.N), sino((size_t)s.N * s.T);
-          ACU_HIP_CHECK(hipMemcpy(sino.p, sino_host, (size_t)s.N * s.T * 4, hipMemcpyHostToDevice));
+          ACU_HIP_CHECK(hipMemcpyAsync(sino.p, sino_host, (size_t)s.N * s.T * 4, hipMemcpyHostToDevice,
+                                       b->ctx->stream));
           acu::ramp_dev(b, 0, s.sinoCS, sino.p);
           acu::bp_dev(b, 0, img.p, s.sinoCS);
-          b->ctx->sync();
-          ACU_HIP_CHECK(hipMemcpy(img_host, img.p, (size_t)s.N * s.N * 4, hipMemcpyDeviceToHost)));
+          ACU_HIP_CHECK(hipMemcpyAsync(img_host, img.p, (size_t)s.N * s.N * 4, hipMemcpyDeviceToHost,
+                                       b->ctx->stream));
+          b->ctx->sync());
 }
 
 int acu_get_slice(acu_batch* b, const float* f_host, float* img_fbp_host) {
@@ -1518,10 +1523,12 @@ int acu_radon_adjoint(acu_batch* b, float* img_host, const float* sino_host) {
   ACU_TRY(hipSetDevice(b->ctx->device);
           acu::Slice& s = acu::slice_of(b, 0);
           DevBuf img((size_t)s.N * s.N), sino((size_t)s.N * s.T);
-          ACU_HIP_CHECK(hipMemcpy(sino.p, sino_host, (size_t)s.N * s.T * 4, hipMemcpyHostToDevice));
+          ACU_HIP_CHECK(hipMemcpyAsync(sino.p, sino_host, (size_t)s.N * s.T * 4, hipMemcpyHostToDevice,
+                                       b->ctx->stream));
           acu::radon_adjoint_dev(b, 0, img.p, sino.p);
-          b->ctx->sync();
-          ACU_HIP_CHECK(hipMemcpy(img_host, img.p, (size_t)s.N * s.N * 4, hipMemcpyDeviceToHost)));
+          ACU_HIP_CHECK(hipMemcpyAsync(img_host, img.p, (size_t)s.N * s.N * 4, hipMemcpyDeviceToHost,
+                                       b->ctx->stream));
+          b->ctx->sync());
 }
 
 namespace {
@@ -1847,8 +1854,8 @@ void find_center(acu_ctx* c, int n, int add_nx, const float* p0h, const float* p
   struct Free { float*& a; float*& b; ~Free() { if (a) hipFree(a); if (b) hipFree(b); } } fr{p0, p180};
   ACU_HIP_CHECK(hipMalloc(&p0, bytes));
   ACU_HIP_CHECK(hipMalloc(&p180, bytes));
-  ACU_HIP_CHECK(hipMemcpy(p0, p0h, bytes, hipMemcpyHostToDevice));
-  ACU_HIP_CHECK(hipMemcpy(p180, p180h, bytes, hipMemcpyHostToDevice));
+  ACU_HIP_CHECK(hipMemcpyAsync(p0, p0h, bytes, hipMemcpyHostToDevice, c->stream));
+  ACU_HIP_CHECK(hipMemcpyAsync(p180, p180h, bytes, hipMemcpyHostToDevice, c->stream));
   float best_shift = centroid_shift(n, add_nx, p0h, p180h), best_theta = 0.0f;
   float dshift = 1e-4f, dtheta = 1e-4f;
   const int n_sp = 6;
