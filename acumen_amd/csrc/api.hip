@@ -154,12 +154,16 @@ template <class T>
 struct DevArray {
   T* d = nullptr;
   int n = 0;
-  void upload(const std::vector<T>& h) {
+  // The copy goes on the context's stream and is waited for (h may be a
+  // temporary): a copy on the legacy null stream fails while another
+  // thread's stream captures a graph.
+  void upload(const std::vector<T>& h, hipStream_t st) {
     if (d) { hipFree(d); d = nullptr; }
     n = (int)h.size();
     if (n == 0) return;
     ACU_HIP_CHECK(hipMalloc(&d, sizeof(T) * h.size()));
-    ACU_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    ACU_HIP_CHECK(hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+    ACU_HIP_CHECK(hipStreamSynchronize(st));
   }
   void release() { if (d) hipFree(d); d = nullptr; n = 0; }
 };
@@ -343,18 +347,21 @@ void upload_tables(acu_batch* b, int i) {
   for (int y = 0; y < s.N; ++y)
     for (int x = 0; x < s.N; ++x)
       if (cm[(size_t)y * s.N + x] != 0.0f) act[(size_t)(y / BP_SUB) * s.ntx + x / BP_TX] = 1;
-  ACU_HIP_CHECK(hipMemcpy(s.h, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  hipStream_t st = b->ctx->stream;
+  ACU_HIP_CHECK(hipMemcpyAsync(s.h, h.data(), h.size() * 4, hipMemcpyHostToDevice, st));
   s.ff_edge = h[2 * s.N - 1];
   if (s.fftG) {
     const int L = fft_len(s.N);
     std::vector<float> G(L);
     std::vector<float2> W(L / 2);
     fft_tables(s.N, h.data(), G.data(), W.data());
-    ACU_HIP_CHECK(hipMemcpy(s.fftG, G.data(), G.size() * 4, hipMemcpyHostToDevice));
-    ACU_HIP_CHECK(hipMemcpy(s.fftW, W.data(), W.size() * sizeof(float2), hipMemcpyHostToDevice));
+    ACU_HIP_CHECK(hipMemcpyAsync(s.fftG, G.data(), G.size() * 4, hipMemcpyHostToDevice, st));
+    ACU_HIP_CHECK(hipMemcpyAsync(s.fftW, W.data(), W.size() * sizeof(float2), hipMemcpyHostToDevice, st));
+    ACU_HIP_CHECK(hipStreamSynchronize(st));  // G and W end here
   }
-  ACU_HIP_CHECK(hipMemcpy(s.cmask, cm.data(), cm.size() * 4, hipMemcpyHostToDevice));
-  ACU_HIP_CHECK(hipMemcpy(s.tact, act.data(), act.size(), hipMemcpyHostToDevice));
+  ACU_HIP_CHECK(hipMemcpyAsync(s.cmask, cm.data(), cm.size() * 4, hipMemcpyHostToDevice, st));
+  ACU_HIP_CHECK(hipMemcpyAsync(s.tact, act.data(), act.size(), hipMemcpyHostToDevice, st));
+  ACU_HIP_CHECK(hipStreamSynchronize(st));
 }
 
 // Tables of slice i from slice j of the same (N, T, wpu), device to device:
@@ -362,14 +369,15 @@ void upload_tables(acu_batch* b, int i) {
 void copy_tables(acu_batch* b, int j, int i) {
   const auto& a = b->s[j];
   auto& s = b->s[i];
-  ACU_HIP_CHECK(hipMemcpy(s.h, a.h, (size_t)(2 * s.N + 1) * 4, hipMemcpyDeviceToDevice));
-  ACU_HIP_CHECK(hipMemcpy(s.cmask, a.cmask, (size_t)s.N * s.N * 4, hipMemcpyDeviceToDevice));
-  ACU_HIP_CHECK(hipMemcpy(s.tact, a.tact, (size_t)s.ntx * s.nsub, hipMemcpyDeviceToDevice));
+  hipStream_t st = b->ctx->stream;  // ordered before every later use of the tables
+  ACU_HIP_CHECK(hipMemcpyAsync(s.h, a.h, (size_t)(2 * s.N + 1) * 4, hipMemcpyDeviceToDevice, st));
+  ACU_HIP_CHECK(hipMemcpyAsync(s.cmask, a.cmask, (size_t)s.N * s.N * 4, hipMemcpyDeviceToDevice, st));
+  ACU_HIP_CHECK(hipMemcpyAsync(s.tact, a.tact, (size_t)s.ntx * s.nsub, hipMemcpyDeviceToDevice, st));
   s.ff_edge = a.ff_edge;
   if (s.fftG) {
     const size_t L = (size_t)fft_len(s.N);
-    ACU_HIP_CHECK(hipMemcpy(s.fftG, a.fftG, L * 4, hipMemcpyDeviceToDevice));
-    ACU_HIP_CHECK(hipMemcpy(s.fftW, a.fftW, L / 2 * sizeof(float2), hipMemcpyDeviceToDevice));
+    ACU_HIP_CHECK(hipMemcpyAsync(s.fftG, a.fftG, L * 4, hipMemcpyDeviceToDevice, st));
+    ACU_HIP_CHECK(hipMemcpyAsync(s.fftW, a.fftW, L / 2 * sizeof(float2), hipMemcpyDeviceToDevice, st));
   }
 }
 
@@ -397,12 +405,14 @@ void upload_theta(acu_batch* b, int i) {
   std::vector<float2> bp(s.T), fp(s.T);
   make_angles(s.T, s.theta.data(), true, bp.data());
   make_angles(s.T, s.theta.data(), false, fp.data());
-  ACU_HIP_CHECK(hipMemcpy(s.scbp, bp.data(), bp.size() * sizeof(float2), hipMemcpyHostToDevice));
-  ACU_HIP_CHECK(hipMemcpy(s.scfp, fp.data(), fp.size() * sizeof(float2), hipMemcpyHostToDevice));
+  hipStream_t st = b->ctx->stream;
+  ACU_HIP_CHECK(hipMemcpyAsync(s.scbp, bp.data(), bp.size() * sizeof(float2), hipMemcpyHostToDevice, st));
+  ACU_HIP_CHECK(hipMemcpyAsync(s.scfp, fp.data(), fp.size() * sizeof(float2), hipMemcpyHostToDevice, st));
   for (int g = 0; g < 3; ++g) {
     const std::vector<uint8_t>& pc = pitch_classes_cached(s.N, s.T, s.theta, 1 << g);
-    ACU_HIP_CHECK(hipMemcpy(s.pcls[g], pc.data(), pc.size(), hipMemcpyHostToDevice));
+    ACU_HIP_CHECK(hipMemcpyAsync(s.pcls[g], pc.data(), pc.size(), hipMemcpyHostToDevice, st));
   }
+  ACU_HIP_CHECK(hipStreamSynchronize(st));  // bp and fp end here
   s.theta_set = true;
 }
 
@@ -439,7 +449,7 @@ template <class T, class F>
 void build(acu_batch* b, DevArray<T>& arr, F&& make) {
   std::vector<T> v;
   for (auto& s : b->s) v.push_back(make(s));
-  arr.upload(v);
+  arr.upload(v, b->ctx->stream);
 }
 
 // Block ranges of the forward-projection groups from each slice's variant:
@@ -474,9 +484,9 @@ void build_fpset(acu_batch* b, FPSet& set, ImgF img, OutF out) {
     if (s.fpv > 0) fl[fl_group(s.fpv)].push_back(fpl_task(s, img(s), out(s)));
     else fp.push_back(fp_task(s, out(s)));
   }
-  set.cl.upload(cl);
-  for (int g = 0; g < FL_GROUPS; ++g) set.fl[g].upload(fl[g]);
-  set.fp.upload(fp);
+  set.cl.upload(cl, b->ctx->stream);
+  for (int g = 0; g < FL_GROUPS; ++g) set.fl[g].upload(fl[g], b->ctx->stream);
+  set.fp.upload(fp, b->ctx->stream);
 }
 
 void build_tv_tasks(acu_batch* b);
@@ -489,8 +499,8 @@ void build_rfset(acu_batch* b, RFSet& set, InF in, OutF out) {
     rf.push_back(rf_task(s, in(s), out(s)));
     if (s.fftG) ff.push_back(ff_task(s, in(s), out(s)));
   }
-  set.rf.upload(rf);
-  set.ff.upload(ff);
+  set.rf.upload(rf, b->ctx->stream);
+  set.ff.upload(ff, b->ctx->stream);
 }
 
 // After a change of variants or angles: new ranges, new task arrays, and a
@@ -564,9 +574,12 @@ void ensure_tv(acu_batch* b, int delta_cap) {
   ACU_HIP_CHECK(hipMalloc(&b->tv_mem, dry.off + 256));
   Carver cv{static_cast<char*>(b->tv_mem), 0};
   layout_tv(b, cv, delta_cap);
-  for (auto& s : b->s) ACU_HIP_CHECK(hipMemset(s.scal, 0, SCAL_STRIDE * sizeof(double)));
+  for (auto& s : b->s)
+    ACU_HIP_CHECK(hipMemsetAsync(s.scal, 0, SCAL_STRIDE * sizeof(double), b->ctx->stream));
   b->tv_delta_cap = delta_cap;
   build_tv_tasks(b);
+  // the fused TV tasks point into the old layout: tvopt rebuilds them
+  b->tv.release();
 }
 
 void require_theta(acu_batch* b) {
@@ -750,7 +763,8 @@ void adhoc_slot(acu_batch* b, size_t bytes) {
 template <class T>
 T* put_adhoc(acu_batch* b, const T& t) {
   adhoc_slot(b, sizeof(T));
-  ACU_HIP_CHECK(hipMemcpy(b->adhoc, &t, sizeof(T), hipMemcpyHostToDevice));
+  ACU_HIP_CHECK(hipMemcpyAsync(b->adhoc, &t, sizeof(T), hipMemcpyHostToDevice, b->ctx->stream));
+  ACU_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));  // t is the caller's temporary
   return static_cast<T*>(b->adhoc);
 }
 
@@ -837,7 +851,7 @@ double time_variant(acu_batch* b, int sl, int v, int reps) {
   DevArray<FPTask> fp;
   CellTask ct = cell_task(s, s.fbp);
   ct.block_begin = 0;
-  cl.upload({ct});
+  cl.upload({ct}, b->ctx->stream);
   int nfl = 0, nfp = 0, nfs = 0, R = 1;
   struct Scratch { float* p = nullptr; ~Scratch() { if (p) hipFree(p); } } sc;
   if (v > 0) {
@@ -851,13 +865,13 @@ double time_variant(acu_batch* b, int sl, int v, int reps) {
       t.block_begin = r * per;
       tv.push_back(t);
     }
-    fl.upload(tv);
+    fl.upload(tv, b->ctx->stream);
     nfl = R * per;
   } else {
     FPTask t = fp_task(s, s.ruk);
     t.block_begin = 0;
     t.sum_begin = 0;
-    fp.upload({t});
+    fp.upload({t}, b->ctx->stream);
     nfp = fp_ray_blocks(s.N) * s.T * fp_chunks(s.N, s.fp_clen);
     nfs = fp_sum_blocks(s.N, s.T);
   }
@@ -952,7 +966,7 @@ double time_bp(acu_batch* b, int ty, int reps) {
     tv.push_back(t);
   }
   DevArray<BPTask> arr;
-  arr.upload(tv);
+  arr.upload(tv, b->ctx->stream);
   auto run = [&] { launch_backproject(arr.d, arr.n, 0, tiles, ty, c->stream); };
   hipEvent_t e0 = c->ev(), e1 = c->ev();
   run();
@@ -1121,13 +1135,13 @@ void loss_grad_run(acu_batch* b, const std::vector<const float*>& x, const std::
       adjb += adj_tiles(s.N) * adj_tiles(s.N);
     }
   }
-  b->res_g.upload(res);
-  b->tvg_g.upload(tvg);
-  b->loss_g.upload(lt);
+  b->res_g.upload(res, b->ctx->stream);
+  b->tvg_g.upload(tvg, b->ctx->stream);
+  b->loss_g.upload(lt, b->ctx->stream);
   launch_shift_resid(b->res_g.d, n, 0, rows, c->stream);
   check_launch();
   if (!adj.empty()) {
-    b->adj_g.upload(adj);
+    b->adj_g.upload(adj, b->ctx->stream);
     timed(c, K_ADJ, [&] { launch_adjoint(b->adj_g.d, n, 0, adjb, c->stream); });
     check_launch();
   }
@@ -1280,7 +1294,7 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       } else {
         acu::copy_tables(b.get(), it->second, i);
       }
-      ACU_HIP_CHECK(hipMemset(b->s[i].f, 0, (size_t)b->s[i].N * b->s[i].T * 4));
+      ACU_HIP_CHECK(hipMemsetAsync(b->s[i].f, 0, (size_t)b->s[i].N * b->s[i].T * 4, b->ctx->stream));
       if (theta_deg && theta_deg[i]) {
         std::memcpy(b->s[i].theta.data(), theta_deg[i], sizeof(float) * b->s[i].T);
         acu::upload_theta(b.get(), i);
