@@ -130,6 +130,9 @@ class Batch:
         return out
 
     def device_ptr(self, i: int, which: int) -> int:
+        """Device address of slice i's f, FBP or TV array.  The TV address
+        moves when a tvopt runs more outer iterations than any before on
+        this batch; ask again after such a call."""
         p = self._lib.acu_batch_device_ptr(self.handle, i, which)
         if not p:
             check(_lib.ACU_ERR_ARG)

@@ -101,7 +101,10 @@ int acu_batch_set_sino(acu_batch *b, int slice, const float *f_host);
 int acu_batch_get_image(acu_batch *b, int slice, int which, float *img_host);
 
 /* Device address of a slice's f, FBP or TV array, for callers that keep
- * their data resident on the GPU.  Returns NULL on error. */
+ * their data resident on the GPU.  Returns NULL on error.  The f and FBP
+ * addresses hold until acu_batch_destroy.  The TV address holds until a
+ * tvopt with more outer iterations than any before on this batch: that
+ * call moves the TV arrays, so ask for the address again afterwards. */
 float *acu_batch_device_ptr(acu_batch *b, int slice, int which);
 
 /* get_slice (fbp_hpc.F90:129-150) for every slice: img_fbp = iradon(f). */
