@@ -71,21 +71,23 @@ void FN(cmask)(int N, int T, int wpu, REAL *cmask)
 
 /* Ramp filtering (first half of iradon), tv_hpc.F90:93-105:
  *   sinoCS(jy,t) = sum_{jx=1..N} sino(jx,t) * filter(jy-jx+1+N)
- * summed sequentially in jx.  The guard jk in (1, 2N+2) is kept. */
+ * summed sequentially in jx.  The reference's guard 1 < jk < 2N+2 on
+ * jk = jy-jx+1+N always holds for jx, jy in 1..N (jk runs over 2..2N), so it
+ * is not tested.  Loop order: jx outer, jy inner, with one running sum per
+ * output jy -- every output is still the sum over jx = 1..N in that order,
+ * one rounding per operation, and the inner loop runs over contiguous taps. */
 void FN(ramp)(int N, int T, const REAL *h, const REAL *sino, REAL *sinoCS)
 {
-    const int nfilt = 2 * N + 1;
 #pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int jt = 1; jt <= T; ++jt) {
         const REAL *s = sino + (size_t)(jt - 1) * N;
-        for (int jy = 1; jy <= N; ++jy) {
-            REAL sum = RC(0.0);
-            for (int jx = 1; jx <= N; ++jx) {
-                const int jk = jy - jx + 1 + N;
-                if (jk > 1 && jk < nfilt + 1)
-                    sum = sum + s[jx - 1] * h[jk - 1];
-            }
-            sinoCS[(jy - 1) + (size_t)(jt - 1) * N] = sum;
+        REAL *out = sinoCS + (size_t)(jt - 1) * N;
+        for (int jy = 0; jy < N; ++jy) out[jy] = RC(0.0);
+        for (int jx = 1; jx <= N; ++jx) {
+            const REAL sv = s[jx - 1];
+            const REAL *hk = h + (N - jx);  /* hk[jy-1] = filter(jy-jx+1+N) */
+            for (int jy = 1; jy <= N; ++jy)
+                out[jy - 1] = out[jy - 1] + sv * hk[jy];
         }
     }
 }
@@ -117,7 +119,15 @@ static void FN(angles)(int T, const float *theta, int bp, REAL *sn, REAL *cs)
  * iradon), tv_hpc.F90:107-123:
  *   yr = sin(c)*(x-m) + cos(c)*(y-m) + m,  y1 = floor(yr), y2 = y1+1,
  *   taps used iff y1 > 1 and y2 < N+1,
- *   img(x,y) = cmask(x,y) * sum_t [S(y1)*(y2-yr) + S(y2)*(yr-y1)]. */
+ *   img(x,y) = cmask(x,y) * sum_t [S(y1)*(y2-yr) + S(y2)*(yr-y1)].
+ * Loop order: blocks of BP_RB image rows x blocks of BP_AB angles, with one
+ * running sum per pixel; angle blocks go in order, so every pixel is still
+ * the sum over t = 1..T in that order (two additions per angle, as in the
+ * reference), while a block of sinogram rows stays in cache. */
+#ifndef ORC_BP_BLOCKS
+#define ORC_BP_BLOCKS
+enum { BP_RB = 8, BP_AB = 32 };
+#endif
 void FN(backproject)(int N, int T, const float *theta, const REAL *cmask,
                      const REAL *sinoCS, REAL *img)
 {
@@ -125,23 +135,44 @@ void FN(backproject)(int N, int T, const float *theta, const REAL *cmask,
     REAL *cs = (REAL *)malloc(sizeof(REAL) * (size_t)T);
     FN(angles)(T, theta, 1, sn, cs);
     const REAL m = FN(midpoint)(N);
-#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
-    for (int jy = 1; jy <= N; ++jy) {
-        for (int jx = 1; jx <= N; ++jx) {
-            REAL sum = RC(0.0);
-            for (int jt = 1; jt <= T; ++jt) {
-                const REAL yr = (sn[jt - 1] * ((REAL)jx - m) + cs[jt - 1] * ((REAL)jy - m)) + m;
-                const int y1 = (int)FLOOR(yr);
-                const int y2 = y1 + 1;
-                if (y1 > 1 && y2 < N + 1) {
-                    const REAL *S = sinoCS + (size_t)(jt - 1) * N;
-                    sum = sum + S[y1 - 1] * ((REAL)y2 - yr);
-                    sum = sum + S[y2 - 1] * (yr - (REAL)y1);
+    const int nrb = (N + BP_RB - 1) / BP_RB;
+#pragma omp parallel num_threads(g_threads) if (g_threads > 1)
+    {
+        REAL *sum = (REAL *)malloc(sizeof(REAL) * (size_t)BP_RB * N);
+        REAL *um = (REAL *)malloc(sizeof(REAL) * (size_t)N);
+        for (int jx = 1; jx <= N; ++jx) um[jx - 1] = (REAL)jx - m;
+#pragma omp for schedule(dynamic, 1)
+        for (int rb = 0; rb < nrb; ++rb) {
+            const int y0 = 1 + rb * BP_RB, y1e = y0 + BP_RB - 1 < N ? y0 + BP_RB - 1 : N;
+            for (size_t i = 0; i < (size_t)BP_RB * N; ++i) sum[i] = RC(0.0);
+            for (int t0 = 1; t0 <= T; t0 += BP_AB) {
+                const int t1 = t0 + BP_AB - 1 < T ? t0 + BP_AB - 1 : T;
+                for (int jy = y0; jy <= y1e; ++jy) {
+                    REAL *row = sum + (size_t)(jy - y0) * N;
+                    const REAL vy = (REAL)jy - m;
+                    for (int jt = t0; jt <= t1; ++jt) {
+                        const REAL *S = sinoCS + (size_t)(jt - 1) * N;
+                        const REAL st = sn[jt - 1], cy = cs[jt - 1] * vy;
+                        for (int jx = 1; jx <= N; ++jx) {
+                            const REAL yr = (st * um[jx - 1] + cy) + m;
+                            const int ya = (int)FLOOR(yr);
+                            const int yb = ya + 1;
+                            if (ya > 1 && yb < N + 1) {
+                                REAL v = row[jx - 1] + S[ya - 1] * ((REAL)yb - yr);
+                                row[jx - 1] = v + S[yb - 1] * (yr - (REAL)ya);
+                            }
+                        }
+                    }
                 }
             }
-            const size_t o = (size_t)(jx - 1) + (size_t)(jy - 1) * N;
-            img[o] = cmask[o] * sum;
+            for (int jy = y0; jy <= y1e; ++jy)
+                for (int jx = 1; jx <= N; ++jx) {
+                    const size_t o = (size_t)(jx - 1) + (size_t)(jy - 1) * N;
+                    img[o] = cmask[o] * sum[(size_t)(jy - y0) * N + (jx - 1)];
+                }
         }
+        free(um);
+        free(sum);
     }
     free(sn);
     free(cs);
@@ -160,39 +191,59 @@ void FN(iradon)(int N, int T, const float *theta, const REAL *h, const REAL *cma
 /* Ray-driven bilinear forward projection, tv_hpc.F90:129-159:
  *   xr = cos(c)*(x-m) - sin(c)*(y-m) + m,  yr = sin(c)*(x-m) + cos(c)*(y-m) + m
  *   taps used iff min(x1,y1) > 1 and max(x2,y2) < N+1,
- *   sino(y,t) = (1/N) * sum_{x=1..N} bilinear(img; xr, yr), x in order. */
+ *   sino(y,t) = (1/N) * sum_{x=1..N} bilinear(img; xr, yr), x in order.
+ * Loop order: per angle, blocks of FP_RB neighbouring rays step together
+ * (x outer, rays inner), one running sum per ray: each ray is still summed
+ * over x = 1..N in that order, while the block's samples at one step are
+ * neighbouring pixels that stay in cache from one step to the next.  The
+ * products c*(x-m), s*(x-m), s*(y-m) and c*(y-m) are the reference's
+ * (each rounded once), computed once and reused. */
+#ifndef ORC_FP_BLOCKS
+#define ORC_FP_BLOCKS
+enum { FP_RB = 16 };
+#endif
 void FN(radon)(int N, int T, const float *theta, const REAL *img, REAL *sino)
 {
     REAL *sn = (REAL *)malloc(sizeof(REAL) * (size_t)T);
     REAL *cs = (REAL *)malloc(sizeof(REAL) * (size_t)T);
     FN(angles)(T, theta, 0, sn, cs);
     const REAL m = FN(midpoint)(N);
-#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(g_threads) if (g_threads > 1)
     for (int jt = 1; jt <= T; ++jt) {
         const REAL s = sn[jt - 1], c = cs[jt - 1];
-        for (int jy = 1; jy <= N; ++jy) {
-            REAL sum = RC(0.0);
+        for (int y0 = 1; y0 <= N; y0 += FP_RB) {
+            const int nr = y0 + FP_RB - 1 <= N ? FP_RB : N - y0 + 1;
+            REAL sum[FP_RB], sy[FP_RB], cy[FP_RB];
+            for (int k = 0; k < nr; ++k) {
+                const REAL v = (REAL)(y0 + k) - m;
+                sum[k] = RC(0.0);
+                sy[k] = s * v;
+                cy[k] = c * v;
+            }
             for (int jx = 1; jx <= N; ++jx) {
-                const REAL xr = (c * ((REAL)jx - m) - s * ((REAL)jy - m)) + m;
-                const REAL yr = (s * ((REAL)jx - m) + c * ((REAL)jy - m)) + m;
-                const int y1 = (int)FLOOR(yr), y2 = y1 + 1;
-                const int x1 = (int)FLOOR(xr), x2 = x1 + 1;
-                const int mn = x1 < y1 ? x1 : y1;
-                const int mx = x2 > y2 ? x2 : y2;
-                if (mn > 1 && mx < N + 1) {
-                    const REAL i11 = img[(x1 - 1) + (size_t)(y1 - 1) * N];
-                    const REAL i21 = img[(x2 - 1) + (size_t)(y1 - 1) * N];
-                    const REAL i12 = img[(x1 - 1) + (size_t)(y2 - 1) * N];
-                    const REAL i22 = img[(x2 - 1) + (size_t)(y2 - 1) * N];
-                    const REAL ax = (REAL)x2 - xr, bx = xr - (REAL)x1;
-                    const REAL ay = (REAL)y2 - yr, by = yr - (REAL)y1;
-                    sum = sum + (i11 * ax) * ay;
-                    sum = sum + (i21 * bx) * ay;
-                    sum = sum + (i12 * ax) * by;
-                    sum = sum + (i22 * bx) * by;
+                const REAL u = (REAL)jx - m;
+                const REAL cu = c * u, su = s * u;
+                for (int k = 0; k < nr; ++k) {
+                    const REAL xr = (cu - sy[k]) + m;
+                    const REAL yr = (su + cy[k]) + m;
+                    const int y1 = (int)FLOOR(yr), y2 = y1 + 1;
+                    const int x1 = (int)FLOOR(xr), x2 = x1 + 1;
+                    const int mn = x1 < y1 ? x1 : y1;
+                    const int mx = x2 > y2 ? x2 : y2;
+                    if (mn > 1 && mx < N + 1) {
+                        const REAL *r1 = img + (size_t)(y1 - 1) * N;
+                        const REAL *r2 = r1 + N;
+                        const REAL ax = (REAL)x2 - xr, bx = xr - (REAL)x1;
+                        const REAL ay = (REAL)y2 - yr, by = yr - (REAL)y1;
+                        REAL v = sum[k] + (r1[x1 - 1] * ax) * ay;
+                        v = v + (r1[x2 - 1] * bx) * ay;
+                        v = v + (r2[x1 - 1] * ax) * by;
+                        sum[k] = v + (r2[x2 - 1] * bx) * by;
+                    }
                 }
             }
-            sino[(jy - 1) + (size_t)(jt - 1) * N] = sum / (REAL)N;
+            for (int k = 0; k < nr; ++k)
+                sino[(size_t)(y0 + k - 1) + (size_t)(jt - 1) * N] = sum[k] / (REAL)N;
         }
     }
     free(sn);
