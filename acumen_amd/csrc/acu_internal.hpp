@@ -120,6 +120,7 @@ struct BPTask {
   const float* cmask;    // N*N
   const float2* sc;      // per-angle (sin, cos) for backprojection
   const uint8_t* tile_active;  // nsub*ntx: 1 if any cmask > 0 in the 64 x 16 sub-tile
+  const float* rowterm;  // T*N (+ padding): cos(c_t) * (y - midpoint) in FP32, angle-major
   float* img;            // N*N out
 };
 
@@ -191,8 +192,15 @@ int fft_len(int N);
 bool fft_supported(int N);
 void fft_tables(int N, const float* taps, float* G, float2* tw);
 void launch_ramp_fft(const FFTTask* d_tasks, int ntasks, int base, int blocks, int maxL, hipStream_t s);
-void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, int ty, hipStream_t s);
+// refpos: positions in the reference's FP32 arithmetic (default); false:
+// tile-relative positions (closer to FP64, two VALU fewer per update)
+void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, int ty, bool refpos,
+                        hipStream_t s);
 void launch_cells(const CellTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
+// rowterm[t*N + y] = cos(c_t) * ((y+1) - mid), one FP32 product each (the
+// backprojector's row term of tv_hpc.F90:114), for t < T, y < N
+constexpr int BP_ROWTERM_PAD = 64;  // floats after the table (the last tile's rows past N)
+void launch_bp_rowterm(const float2* sc, int N, int T, float mid, float* rowterm, hipStream_t s);
 void launch_forward(const FPTask* d_tasks, int ntasks, int base, int blocks, int sum_base,
                     int sum_blocks, hipStream_t s);
 // al = 1, 2, 4: fpl.hip; al = FPV_CELL: fplc.hip
@@ -238,4 +246,13 @@ struct Error {
   std::string msg;
   explicit Error(std::string m) : msg(std::move(m)) {}
 };
+}  // namespace acu
+
+// Context internals for entry points defined outside api.hip.
+struct acu_ctx;
+namespace acu {
+int ctx_device(const acu_ctx* c);
+hipStream_t ctx_stream(const acu_ctx* c);
+// Record msg as the calling thread's acu_last_error text; returns code.
+int record_error(int code, const std::string& msg);
 }  // namespace acu

@@ -177,6 +177,7 @@ struct Slice {
   // operator arena
   float *h = nullptr, *cmask = nullptr;
   float2 *scbp = nullptr, *scfp = nullptr;
+  float* rowterm = nullptr;  // backprojection row terms cos(c_t) * (y - mid), T*N + pad
   uint8_t* tact = nullptr;
   float *f = nullptr, *sinoCS = nullptr, *fbp = nullptr, *ruk = nullptr, *rp = nullptr;
   float4 *cells = nullptr, *cellsT = nullptr;
@@ -191,6 +192,7 @@ struct Slice {
   float *uk = nullptr, *ukp = nullptr, *dkx = nullptr, *dky = nullptr, *bkx = nullptr,
         *bky = nullptr, *ifkt = nullptr, *iukt = nullptr, *r = nullptr, *p = nullptr,
         *irpt = nullptr, *Ap = nullptr, *fk = nullptr;
+  double *ifkt64 = nullptr, *iukt64 = nullptr, *ruk64 = nullptr;
   double *scal = nullptr, *partials = nullptr;
   float* delta = nullptr;
   int delta_cap = 0;
@@ -221,7 +223,7 @@ struct RFSet {
 // slice appears in the cells array only when its variant reads the cells.
 struct FPSet {
   DevArray<CellTask> cl;
-  DevArray<FPLTask> fl[4];
+  DevArray<FPLTask> fl[4];  // one per LDS variant group (FL_GROUPS)
   DevArray<FPTask> fp;
   void release() {
     cl.release();
@@ -233,6 +235,8 @@ struct FPSet {
 // task group of an LDS projector variant: al = 1, 2, 4, FPV_CELL -> 0..3
 inline int fl_group(int al) { return al == FPV_CELL ? 3 : al >= 4 ? 2 : al == 2 ? 1 : 0; }
 constexpr int FL_GROUPS = 4;
+static_assert(sizeof(FPSet::fl) / sizeof(FPSet::fl[0]) == FL_GROUPS, "one task array per variant group");
+inline bool fpv_valid(int v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == FPV_CELL; }
 
 }  // namespace
 }  // namespace acu
@@ -246,9 +250,10 @@ struct acu_batch {
   int ff_total = 0, ff_maxL = 0;  // FFT ramp blocks and the largest transform
   int rf_total = 0, bp_total = 0, cl_total = 0, fp_total = 0, fs_total = 0, ew_total = 0,
       sw_total = 0;
-  int fl_total[4] = {0, 0, 0, 0};  // LDS projector blocks per variant group (lds1, lds2, lds4, cell16)
+  int fl_total[acu::FL_GROUPS] = {0, 0, 0, 0};  // LDS projector blocks per variant group (lds1, lds2, lds4, cell16)
   int ramp_variant = acu::RAMP_FP64;  // arithmetic of the ramp filter
   int bp_ty = acu::BP_TY_DEFAULT;     // backprojection tile height (16, 32 or 64)
+  bool bp_refpos = true;              // backprojection positions in the reference's arithmetic
   // task sets for the fixed operand combinations of FBP / TV
   acu::RFSet rf_f, rf_fk, rf_ruk, rf_rp;  // ramp filter of f, fk, ruk, rp
   acu::DevArray<acu::BPTask> bp_fbp, bp_ifkt, bp_iukt, bp_irpt;
@@ -264,12 +269,25 @@ struct acu_batch {
   acu::DevArray<acu::AdjTask> adj_g;
   acu::DevArray<acu::TVGTask> tvg_g;
   acu::DevArray<acu::LossTask> loss_g;
-  // graph of one outer iteration
-  hipGraphExec_t graph = nullptr;
-  int graph_ninner = -1, graph_ncg = -1;
+  // operator schedule of an outer iteration (ACU_TV_SCHED_*) and, for the
+  // linear schedule, the exact refresh period in outer iterations (0: none)
+  int tv_sched = ACU_TV_SCHED_LINEAR;
+  int tv_refresh = 0;
+  // last tvopt's outer-iteration count (the length of its delta log)
+  int last_nouter = -1;
+  // graphs of one outer iteration: [0] the regular body, [1] the linear
+  // schedule's refresh body
+  hipGraphExec_t graph = nullptr, graph_refresh = nullptr;
+  int graph_ninner = -1, graph_ncg = -1, graph_sched = -1;
+
+  void drop_graphs() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (graph_refresh) (void)hipGraphExecDestroy(graph_refresh);
+    graph = graph_refresh = nullptr;
+  }
 
   ~acu_batch() {
-    if (graph) hipGraphExecDestroy(graph);
+    drop_graphs();
     rf_f.release(); rf_fk.release(); rf_ruk.release(); rf_rp.release();
     bp_fbp.release(); bp_ifkt.release(); bp_iukt.release(); bp_irpt.release();
     fp_uk.release(); fp_p.release(); tv.release();
@@ -307,6 +325,7 @@ void layout_ops(acu_batch* b, Carver& cv) {
     s.cmask = cv.take<float>(NN);
     s.scbp = cv.take<float2>(s.T);
     s.scfp = cv.take<float2>(s.T);
+    s.rowterm = cv.take<float>(NT + BP_ROWTERM_PAD);
     s.tact = cv.take<uint8_t>((size_t)s.ntx * s.nsub);
     s.f = cv.take<float>(NT);
     s.sinoCS = cv.take<float>(NT);
@@ -331,6 +350,10 @@ void layout_tv(acu_batch* b, Carver& cv, int delta_cap) {
                       &s.ifkt, &s.iukt, &s.r, &s.p, &s.irpt, &s.Ap};
     for (float** a : imgs) *a = cv.take<float>(NN);
     s.fk = cv.take<float>(NT);
+    // the linear schedule's carried terms, FP64
+    s.ifkt64 = cv.take<double>(NN);
+    s.iukt64 = cv.take<double>(NN);
+    s.ruk64 = cv.take<double>(NT);
     s.scal = cv.take<double>(SCAL_STRIDE);
     s.partials = cv.take<double>((size_t)s.ew_n);
     s.delta = cv.take<float>((size_t)delta_cap);
@@ -412,6 +435,8 @@ void upload_theta(acu_batch* b, int i) {
     const std::vector<uint8_t>& pc = pitch_classes_cached(s.N, s.T, s.theta, 1 << g);
     ACU_HIP_CHECK(hipMemcpyAsync(s.pcls[g], pc.data(), pc.size(), hipMemcpyHostToDevice, st));
   }
+  launch_bp_rowterm(s.scbp, s.N, s.T, s.mid, s.rowterm, st);
+  ACU_HIP_CHECK(hipGetLastError());
   ACU_HIP_CHECK(hipStreamSynchronize(st));  // bp and fp end here
   s.theta_set = true;
 }
@@ -426,7 +451,7 @@ FFTTask ff_task(const Slice& s, const float* in, float* out) {
   return FFTTask{s.N, s.T, L, lg, s.ff_begin, L == 2 * s.N - 2 ? 1 : 0, s.ff_edge, s.fftG, s.fftW, in, out};
 }
 BPTask bp_task(const Slice& s, const float* in, float* out) {
-  return BPTask{s.N, s.T, s.mid, s.bp_begin, s.ntx, s.nty, s.nsub, in, s.cmask, s.scbp, s.tact, out};
+  return BPTask{s.N, s.T, s.mid, s.bp_begin, s.ntx, s.nty, s.nsub, in, s.cmask, s.scbp, s.tact, s.rowterm, out};
 }
 FPLTask fpl_task(const Slice& s, const float* img, float* out) {
   static const bool fixed_pitch = [] {
@@ -508,7 +533,7 @@ void build_rfset(acu_batch* b, RFSet& set, InF in, OutF out) {
 void refresh_fp(acu_batch* b) {
   assign_fp(b);
   if (b->tv_mem) build_tv_tasks(b);
-  if (b->graph) { ACU_HIP_CHECK(hipGraphExecDestroy(b->graph)); b->graph = nullptr; }
+  b->drop_graphs();
 }
 
 // Backprojection tile ranges at the batch's tile height.
@@ -530,7 +555,7 @@ void set_bp_tile(acu_batch* b, int ty) {
   assign_bp(b);
   build_op_tasks(b);
   if (b->tv_mem) build_tv_tasks(b);
-  if (b->graph) { ACU_HIP_CHECK(hipGraphExecDestroy(b->graph)); b->graph = nullptr; }
+  b->drop_graphs();
 }
 
 void build_op_tasks(acu_batch* b) {
@@ -558,7 +583,8 @@ void build_tv_task_array(acu_batch* b) {
     t.mu = b->tv_mu; t.lambda = b->tv_lambda;
     t.uk = s.uk; t.ukp = s.ukp; t.dkx = s.dkx; t.dky = s.dky; t.bkx = s.bkx; t.bky = s.bky;
     t.ifkt = s.ifkt; t.iukt = s.iukt; t.r = s.r; t.p = s.p; t.irpt = s.irpt; t.Ap = s.Ap;
-    t.f = s.f; t.fk = s.fk; t.ruk = s.ruk;
+    t.f = s.f; t.fk = s.fk; t.ruk = s.ruk; t.rp = s.rp; t.fbp = s.fbp;
+    t.ifkt64 = s.ifkt64; t.iukt64 = s.iukt64; t.ruk64 = s.ruk64;
     t.scal = s.scal; t.partials = s.partials;
     t.delta_log = s.delta; t.delta_cap = s.delta_cap;
     return t;
@@ -568,7 +594,7 @@ void build_tv_task_array(acu_batch* b) {
 void ensure_tv(acu_batch* b, int delta_cap) {
   if (b->tv_mem && delta_cap <= b->tv_delta_cap) return;
   if (b->tv_mem) { hipFree(b->tv_mem); b->tv_mem = nullptr; }
-  if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+  b->drop_graphs();
   Carver dry;
   layout_tv(b, dry, delta_cap);
   ACU_HIP_CHECK(hipMalloc(&b->tv_mem, dry.off + 256));
@@ -609,7 +635,7 @@ void iradon_seq(acu_batch* b, const RFSet& rf, const DevArray<BPTask>& bp, int s
   acu_ctx* c = b->ctx;
   const Range r2 = bp_range(b, sl);
   ramp_seq(b, rf, sl);
-  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, b->bp_ty, c->stream); });
+  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, b->bp_ty, b->bp_refpos, c->stream); });
   check_launch();
 }
 
@@ -656,22 +682,41 @@ double read_scalar(acu_batch* b, int sl, int idx) {
 }
 
 // One outer iteration (tv_hpc.F90:205-353) for all slices or one slice.
-// reuse_ruk: ruk already holds radon(uk) for the current uk (true after the
-// prologue and after every add-back, tv_hpc.F90:339, when ninner == 1 or for
-// the first inner iteration), so the radon of :228 is the same result.
+// Operator schedule (b->tv_sched), per outer iteration with ninner = 1:
+//  * REFERENCE: the reference's 7 radon + 7 iradon (:215, :228, :229, five
+//    CG pairs, :339).
+//  * REUSE: 6 + 7.  ruk_fresh says ruk already holds radon(uk) for the
+//    current uk (after the prologue and after every add-back, :339), so the
+//    radon of :228 in the first inner iteration is that same result, bit
+//    for bit.
+//  * LINEAR: 5 + 5.  R and F are linear, and the CG already forms
+//    rp_j = R p_j and irpt_j = F R p_j, so with uk += sum_j alpha_j p_j:
+//      ruk  = R uk   is carried as ruk  += alpha_j rp_j    (replaces :339, :228),
+//      iukt = F R uk is carried as iukt += alpha_j irpt_j  (replaces :229),
+//      ifkt = F fk   is carried as ifkt  = (ifkt + F f) - iukt after the
+//                    add-back fk = (fk + f) - ruk          (replaces :215),
+//    from exact values formed in the prologue.  The three are carried in
+//    FP64 (ruk64, iukt64, ifkt64): carried in FP32, their rounding drifted
+//    to 2e-4 from the FP64 result after 50 outer iterations at 513^2 x 360
+//    (5.7e-5 in FP64, as the reference's own FP32).  refresh: recompute the
+//    three exactly at the start of this outer iteration.
 // host_sync: return every reduction to the host before the next kernel, as
 // the reference does (the plain per-slice path).
-void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool reuse_ruk, bool host_sync) {
+void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool ruk_fresh, bool host_sync, bool refresh) {
   acu_ctx* c = b->ctx;
   const Range ew = ew_range(b, sl), sw = sw_range(b, sl);
   const TVTask* tv = b->tv.d;
   const int nt = b->tv.n;
   const int first = sl < 0 ? 0 : sl, count = sl < 0 ? nt : 1;  // finalize: one block per slice
+  const bool lin = b->tv_sched == ACU_TV_SCHED_LINEAR;
   for (int ji = 0; ji < ninner; ++ji) {
-    iradon_seq(b, b->rf_fk, b->bp_ifkt, sl);                       // :215
-    if (!(reuse_ruk && ji == 0)) radon_seq(b, b->fp_uk, sl);  // :228
-    iradon_seq(b, b->rf_ruk, b->bp_iukt, sl);                      // :229
-    launch_tv_residual(tv, nt, ew.base, ew.n, c->stream);          // :207-249
+    const bool exact = !lin || (refresh && ji == 0);
+    if (exact) iradon_seq(b, b->rf_fk, b->bp_ifkt, sl);            // :215
+    const bool reuse = b->tv_sched == ACU_TV_SCHED_REUSE && ruk_fresh && ji == 0;
+    if (exact && !reuse) radon_seq(b, b->fp_uk, sl);               // :228
+    if (exact) iradon_seq(b, b->rf_ruk, b->bp_iukt, sl);           // :229
+    if (lin && exact) launch_tv_linload(tv, nt, ew.base, ew.n, sw.base, sw.n, c->stream);
+    launch_tv_residual(tv, nt, ew.base, ew.n, lin, c->stream);     // :207-249
     launch_tv_finalize(tv, first, count, 0, c->stream);
     check_launch();
     if (host_sync) (void)read_scalar(b, sl, 0);
@@ -682,25 +727,73 @@ void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool reuse_ruk, bool 
       launch_tv_finalize(tv, first, count, 64 + j, c->stream);
       check_launch();
       if (host_sync) (void)read_scalar(b, sl, 64 + j);
-      launch_tv_axpy(tv, nt, ew.base, ew.n, j, c->stream);
+      launch_tv_axpy(tv, nt, ew.base, ew.n, j, lin, c->stream);
+      if (lin) launch_tv_sino_axpy(tv, nt, sw.base, sw.n, j, c->stream);
       launch_tv_finalize(tv, first, count, j + 1, c->stream);
       check_launch();
       if (host_sync) (void)read_scalar(b, sl, j + 1);
       launch_tv_pupdate(tv, nt, ew.base, ew.n, j, c->stream);
       check_launch();
     }
-    launch_tv_shrink(tv, nt, ew.base, ew.n, c->stream);           // :305-333
+    launch_tv_shrink(tv, nt, ew.base, ew.n, lin && ji == ninner - 1, c->stream);  // :305-333
     launch_tv_finalize(tv, first, count, 127, c->stream);
     check_launch();
     if (host_sync) (void)read_scalar(b, sl, 127);
   }
-  radon_seq(b, b->fp_uk, sl);                                      // :339
-  launch_tv_addback(tv, nt, sw.base, sw.n, c->stream);             // :341-346
+  if (!lin) radon_seq(b, b->fp_uk, sl);                            // :339
+  launch_tv_addback(tv, nt, sw.base, sw.n, lin, c->stream);        // :341-346
   check_launch();
 }
 
 void fbp_all(acu_batch* b, int sl) {
   iradon_seq(b, b->rf_f, b->bp_fbp, sl);
+}
+
+// Prologue of tvopt (tv_hpc.F90:176-202) for all slices or one: FBP warm
+// start, uk = fbp, d = b = 0, fk = f; then what the schedule carries:
+// ruk = radon(uk) (REUSE, LINEAR), iukt = iradon(ruk) and ifkt = iradon(f),
+// which is the FBP itself (LINEAR).
+void tv_prologue(acu_batch* b, int sl, int nouter) {
+  acu_ctx* c = b->ctx;
+  fbp_all(b, sl);
+  const int i0 = sl < 0 ? 0 : sl, i1 = sl < 0 ? (int)b->s.size() : sl + 1;
+  const bool lin = b->tv_sched == ACU_TV_SCHED_LINEAR;
+  for (int i = i0; i < i1; ++i) {
+    const Slice& s = b->s[i];
+    const size_t bytes = (size_t)s.N * s.N * 4;
+    ACU_HIP_CHECK(hipMemcpyAsync(s.uk, s.fbp, bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (lin && nouter > 0) ACU_HIP_CHECK(hipMemcpyAsync(s.ifkt, s.fbp, bytes, hipMemcpyDeviceToDevice, c->stream));
+  }
+  const Range ew = ew_range(b, sl), sw = sw_range(b, sl);
+  launch_tv_init(b->tv.d, b->tv.n, ew.base, ew.n, sw.base, sw.n, c->stream);
+  check_launch();
+  if (nouter > 0 && b->tv_sched != ACU_TV_SCHED_REFERENCE) radon_seq(b, b->fp_uk, sl);
+  if (nouter > 0 && lin) {
+    iradon_seq(b, b->rf_ruk, b->bp_iukt, sl);
+    launch_tv_linload(b->tv.d, b->tv.n, ew.base, ew.n, sw.base, sw.n, c->stream);
+    check_launch();
+  }
+}
+
+bool refresh_at(const acu_batch* b, int jo) {
+  return b->tv_sched == ACU_TV_SCHED_LINEAR && b->tv_refresh > 0 && jo > 0 && jo % b->tv_refresh == 0;
+}
+
+hipGraphExec_t capture_outer(acu_batch* b, int ninner, int ncg, bool refresh) {
+  acu_ctx* c = b->ctx;
+  hipGraph_t g;
+  ACU_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    outer_body(b, -1, ninner, ncg, true, false, refresh);
+  } catch (...) {
+    (void)hipStreamEndCapture(c->stream, &g);
+    throw;
+  }
+  ACU_HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+  hipGraphExec_t e = nullptr;
+  ACU_HIP_CHECK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+  ACU_HIP_CHECK(hipGraphDestroy(g));
+  return e;
 }
 
 void tvopt_run(acu_batch* b, float mu, float lambda, int nouter, int ninner, int ncg, int mode) {
@@ -712,45 +805,42 @@ void tvopt_run(acu_batch* b, float mu, float lambda, int nouter, int ninner, int
     b->tv_mu = mu;
     b->tv_lambda = lambda;
     build_tv_task_array(b);
-    if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    b->drop_graphs();
   }
-  // prologue (tv_hpc.F90:176-202): FBP warm start, uk = fbp, d = b = 0, fk = f
-  fbp_all(b, -1);
-  for (auto& s : b->s)
-    ACU_HIP_CHECK(hipMemcpyAsync(s.uk, s.fbp, (size_t)s.N * s.N * 4, hipMemcpyDeviceToDevice, c->stream));
-  const Range ew = ew_range(b, -1), sw = sw_range(b, -1);
-  launch_tv_init(b->tv.d, b->tv.n, ew.base, ew.n, sw.base, sw.n, c->stream);
-  check_launch();
-  for (auto& s : b->s) { s.fbp_valid = true; s.tv_valid = false; }
+  for (auto& s : b->s) { s.fbp_valid = false; s.tv_valid = false; }
+  b->last_nouter = -1;
   if (mode == ACU_TV_PLAIN) {
-    for (int sl = 0; sl < (int)b->s.size(); ++sl)
-      for (int jo = 0; jo < nouter; ++jo) outer_body(b, sl, ninner, ncg, false, true);
+    // the reference's structure: one slice after another, every kernel of
+    // the slice launched for it alone, every reduction back on the host
+    for (int sl = 0; sl < (int)b->s.size(); ++sl) {
+      tv_prologue(b, sl, nouter);
+      for (int jo = 0; jo < nouter; ++jo) outer_body(b, sl, ninner, ncg, true, true, refresh_at(b, jo));
+    }
   } else {
-    if (nouter > 0) radon_seq(b, b->fp_uk, -1);  // ruk = radon(uk) for :228
+    tv_prologue(b, -1, nouter);
     const bool graph = (mode == ACU_TV_GRAPH) && !c->timing;
     if (!graph) {
-      for (int jo = 0; jo < nouter; ++jo) outer_body(b, -1, ninner, ncg, true, false);
+      for (int jo = 0; jo < nouter; ++jo) outer_body(b, -1, ninner, ncg, true, false, refresh_at(b, jo));
     } else if (nouter > 0) {
-      if (!b->graph || b->graph_ninner != ninner || b->graph_ncg != ncg) {
-        if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
-        hipGraph_t g;
-        ACU_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        try {
-          outer_body(b, -1, ninner, ncg, true, false);
-        } catch (...) {
-          hipStreamEndCapture(c->stream, &g);
-          throw;
-        }
-        ACU_HIP_CHECK(hipStreamEndCapture(c->stream, &g));
-        ACU_HIP_CHECK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
-        ACU_HIP_CHECK(hipGraphDestroy(g));
+      if (!b->graph || b->graph_ninner != ninner || b->graph_ncg != ncg || b->graph_sched != b->tv_sched) {
+        b->drop_graphs();
+        b->graph = capture_outer(b, ninner, ncg, false);
         b->graph_ninner = ninner;
         b->graph_ncg = ncg;
+        b->graph_sched = b->tv_sched;
       }
-      for (int jo = 0; jo < nouter; ++jo) ACU_HIP_CHECK(hipGraphLaunch(b->graph, c->stream));
+      for (int jo = 0; jo < nouter; ++jo) {
+        if (refresh_at(b, jo)) {
+          if (!b->graph_refresh) b->graph_refresh = capture_outer(b, ninner, ncg, true);
+          ACU_HIP_CHECK(hipGraphLaunch(b->graph_refresh, c->stream));
+        } else {
+          ACU_HIP_CHECK(hipGraphLaunch(b->graph, c->stream));
+        }
+      }
     }
   }
-  for (auto& s : b->s) s.tv_valid = true;
+  for (auto& s : b->s) { s.fbp_valid = true; s.tv_valid = true; }
+  b->last_nouter = nouter;
 }
 
 void adhoc_slot(acu_batch* b, size_t bytes) {
@@ -822,7 +912,7 @@ void bp_dev(acu_batch* b, int sl, float* img, const float* filt) {
   BPTask t = bp_task(s, filt, img);
   t.tile_begin = 0;
   BPTask* d = put_adhoc(b, t);
-  timed(c, K_BP, [&] { launch_backproject(d, 1, 0, s.bp_n, b->bp_ty, c->stream); });
+  timed(c, K_BP, [&] { launch_backproject(d, 1, 0, s.bp_n, b->bp_ty, b->bp_refpos, c->stream); });
   check_launch();
 }
 
@@ -967,7 +1057,7 @@ double time_bp(acu_batch* b, int ty, int reps) {
   }
   DevArray<BPTask> arr;
   arr.upload(tv, b->ctx->stream);
-  auto run = [&] { launch_backproject(arr.d, arr.n, 0, tiles, ty, c->stream); };
+  auto run = [&] { launch_backproject(arr.d, arr.n, 0, tiles, ty, b->bp_refpos, c->stream); };
   hipEvent_t e0 = c->ev(), e1 = c->ev();
   run();
   ACU_HIP_CHECK(hipEventRecord(e0, c->stream));
@@ -1169,6 +1259,12 @@ using acu::Error;
     return fail(ACU_ERR_HIP, "host allocation failed");                      \
   }
 
+namespace acu {
+int ctx_device(const acu_ctx* c) { return c->device; }
+hipStream_t ctx_stream(const acu_ctx* c) { return c->stream; }
+int record_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace acu
+
 extern "C" {
 
 const char* acu_last_error(void) { return g_err.c_str(); }
@@ -1252,11 +1348,22 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       b->ramp_variant = r == "fp32" ? acu::RAMP_FP32 : r == "bf16x3" ? acu::RAMP_BF16X3
                         : r == "bf16x6" ? acu::RAMP_BF16X6 : r == "fft" ? acu::RAMP_FFT : acu::RAMP_FP64;
     }
+    // backprojection positions: ACUMEN_BP_POSITIONS = reference (default) | tile
+    if (const char* pv2 = std::getenv("ACUMEN_BP_POSITIONS")) b->bp_refpos = std::strcmp(pv2, "tile") != 0;
     // backprojection tile height: ACUMEN_BP_TILE = 16 | 32 (default) | 64
     if (const char* tv = std::getenv("ACUMEN_BP_TILE")) {
       const int ty = std::atoi(tv);
       if (acu::bp_tile_ok(ty)) b->bp_ty = ty;
     }
+    // operator schedule of tvopt: ACUMEN_TV_SCHEDULE = linear (default) |
+    // reuse | reference; ACUMEN_TV_REFRESH = exact refresh period of the
+    // linear schedule in outer iterations (default 0: none)
+    if (const char* sv = std::getenv("ACUMEN_TV_SCHEDULE")) {
+      const std::string v(sv);
+      b->tv_sched = v == "reference" ? ACU_TV_SCHED_REFERENCE : v == "reuse" ? ACU_TV_SCHED_REUSE
+                                                                             : ACU_TV_SCHED_LINEAR;
+    }
+    if (const char* rv = std::getenv("ACUMEN_TV_REFRESH")) b->tv_refresh = std::max(0, std::atoi(rv));
     // tuning: forward-projection sweep chunk length (steps), ACUMEN_FP_CHUNK
     const char* ev = std::getenv("ACUMEN_FP_CHUNK");
     const int fp_clen_env = ev ? std::atoi(ev) : 0;
@@ -1271,7 +1378,7 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       b->ff_maxL = std::max(b->ff_maxL, acu::fft_len(s.N));
       s.fp_clen = fp_clen_env > 0 ? fp_clen_env : acu::FP_CHUNK_DEFAULT;
       s.fpv = gather ? 0
-                     : (al_env == 1 || al_env == 2 || al_env == 4 || al_env == acu::FPV_CELL) ? al_env
+                     : (al_env > 0 && acu::fpv_valid(al_env)) ? al_env
                                                                    : acu::fpl_angles_per_lane(s.N, s.T);
       s.ew_begin = ew; s.ew_n = acu::ew_blocks((size_t)s.N * s.N); ew += s.ew_n;
       s.sw_begin = sw; s.sw_n = acu::ew_blocks((size_t)s.N * s.T); sw += s.sw_n;
@@ -1332,6 +1439,9 @@ int acu_batch_nslices(const acu_batch* b) { return b ? (int)b->s.size() : 0; }
 int acu_batch_set_theta(acu_batch* b, int slice, const float* theta_deg) {
   if (!b || !theta_deg || slice < 0 || slice >= (int)b->s.size())
     return fail(ACU_ERR_ARG, "acu_batch_set_theta: bad argument");
+  // results of the old angles no longer describe this slice
+  b->s[slice].fbp_valid = false;
+  b->s[slice].tv_valid = false;
   ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync();
           std::memcpy(b->s[slice].theta.data(), theta_deg, sizeof(float) * b->s[slice].T);
           acu::upload_theta(b, slice);
@@ -1343,6 +1453,9 @@ int acu_batch_set_sino(acu_batch* b, int slice, const float* f_host) {
   if (!b || !f_host || slice < 0 || slice >= (int)b->s.size())
     return fail(ACU_ERR_ARG, "acu_batch_set_sino: bad argument");
   auto& s = b->s[slice];
+  // results of the old sinogram no longer describe this slice
+  s.fbp_valid = false;
+  s.tv_valid = false;
   ACU_TRY(hipSetDevice(b->ctx->device);
           ACU_HIP_CHECK(hipMemcpyAsync(s.f, f_host, (size_t)s.N * s.T * 4, hipMemcpyHostToDevice,
                                        b->ctx->stream));
@@ -1397,11 +1510,30 @@ int acu_batch_tvopt(acu_batch* b, float mu, float lambda, int nouter, int ninner
   ACU_TRY(hipSetDevice(b->ctx->device); acu::tvopt_run(b, mu, lambda, nouter, ninner, ncg, mode));
 }
 
+int acu_batch_set_tv_schedule(acu_batch* b, int sched, int refresh) {
+  if (!b) return fail(ACU_ERR_ARG, "null batch");
+  if (sched != ACU_TV_SCHED_REFERENCE && sched != ACU_TV_SCHED_REUSE && sched != ACU_TV_SCHED_LINEAR)
+    return fail(ACU_ERR_ARG, "acu_batch_set_tv_schedule: unknown schedule");
+  if (refresh < 0) return fail(ACU_ERR_ARG, "acu_batch_set_tv_schedule: refresh must be >= 0");
+  ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync(); b->tv_sched = sched; b->tv_refresh = refresh;
+          b->drop_graphs(););
+}
+
+int acu_batch_tv_schedule(const acu_batch* b, int* refresh) {
+  if (!b) return -fail(ACU_ERR_ARG, "null batch");
+  if (refresh) *refresh = b->tv_refresh;
+  return b->tv_sched;
+}
+
 int acu_batch_get_delta(acu_batch* b, int slice, float* delta_host, int n) {
   if (!b || !delta_host || slice < 0 || slice >= (int)b->s.size() || n < 0)
     return fail(ACU_ERR_ARG, "acu_batch_get_delta: bad argument");
   auto& s = b->s[slice];
-  if (!s.delta || n > s.delta_cap) return fail(ACU_ERR_STATE, "no delta log of that length");
+  if (!s.delta || !s.tv_valid || b->last_nouter < 0)
+    return fail(ACU_ERR_STATE, "no tvopt has completed on this slice since its sinogram or angles were set");
+  if (n > b->last_nouter)
+    return fail(ACU_ERR_ARG, "acu_batch_get_delta: n exceeds the last tvopt's nouter (" +
+                                 std::to_string(b->last_nouter) + ")");
   ACU_TRY(hipSetDevice(b->ctx->device);
           ACU_HIP_CHECK(hipMemcpyAsync(delta_host, s.delta, (size_t)n * 4, hipMemcpyDeviceToHost,
                                        b->ctx->stream));
@@ -1495,7 +1627,7 @@ int acu_batch_set_ramp_variant(acu_batch* b, int variant) {
         return fail(ACU_ERR_ARG, "acu_batch_set_ramp_variant: the FFT filter supports N <= " +
                                      std::to_string(acu::FFT_MAX_L / 2 + 1));
   ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync(); b->ramp_variant = variant;
-          if (b->graph) { ACU_HIP_CHECK(hipGraphExecDestroy(b->graph)); b->graph = nullptr; });
+          b->drop_graphs(););
 }
 
 int acu_batch_ramp_variant(const acu_batch* b) {
@@ -1522,7 +1654,7 @@ int acu_batch_fp_variant(const acu_batch* b, int slice) {
 int acu_batch_set_fp_variant(acu_batch* b, int slice, int variant) {
   if (!b || slice < 0 || slice >= (int)b->s.size())
     return fail(ACU_ERR_ARG, "acu_batch_set_fp_variant: bad slice");
-  if (variant != 0 && variant != 1 && variant != 2 && variant != 4 && variant != acu::FPV_CELL)
+  if (!acu::fpv_valid(variant))
     return fail(ACU_ERR_ARG, "acu_batch_set_fp_variant: variant must be 0, 1, 2, 4 or 8");
   ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync(); b->s[slice].fpv = variant; acu::refresh_fp(b));
 }

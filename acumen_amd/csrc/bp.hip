@@ -20,12 +20,19 @@
 //    first is used;
 //  * a pair whose lower tap y1 violates the reference's bounds test is
 //    staged as (0, 0), which reproduces the test exactly with no branch;
-//  * positions are taken relative to the tile origin: yr - 1 - lo =
-//    s*xl + c*yl + K with xl < 64, yl < TY and K = s*(x0+1-m) + c*(y0+1-m)
-//    + m-1-lo formed in FP64 once per angle and tile, so the FP32 rounding
-//    of a position is that of a number below ~100 instead of one near N (the
-//    reference rounds s*(x-m) + c*(y-m) + m at magnitude N, ~1e-4 px at
-//    N = 2049); each 32-angle block is summed separately and then added;
+//  * positions (REFPOS, the default): the reference's own FP32 value,
+//    yr = (s*(x-m) + c*(y-m)) + m with each product and sum rounded once and
+//    no contraction, so every floor, weight and bounds decision is the
+//    reference's.  Its rounding at magnitude N (~1e-4 px at N = 2049) is
+//    what dominates the reference's distance from FP64 on noisy sinograms:
+//    with these positions the FBP is ~1e-6 from the reference's FP32 FBP,
+//    with exact positions ~2.7e-5 (noisy 2049^2 x 1500);
+//  * positions (!REFPOS, ACUMEN_BP_POSITIONS=tile): relative to the tile
+//    origin, yr - 1 - lo = s*xl + c*yl + K with xl < 64, yl < TY and
+//    K = s*(x0+1-m) + c*(y0+1-m) + m-1-lo formed in FP64 once per angle and
+//    tile, so a position is rounded at magnitude below ~100: closer to FP64,
+//    farther from the reference, two VALU fewer per update;
+//  * each 32-angle block is summed separately and then added;
 //  * tiles whose mask is identically zero (by 64 x 16 sub-tiles) only store
 //    zeros.
 #include "acu_internal.hpp"
@@ -52,7 +59,7 @@ __device__ __forceinline__ int find_task(const BPTask* tasks, int ntasks, int gt
   return lo;
 }
 
-template <int TY>
+template <int TY, bool REFPOS>
 __global__ __launch_bounds__(64 * bp_waves(TY)) void bp_kernel(const BPTask* __restrict__ tasks, int ntasks, int base) {
   constexpr int NW = bp_waves(TY), NT = 64 * NW, PY = TY / NW, W = bp_window(TY), SUBS = TY / BP_SUB;
   constexpr int SPA = NT / AB, NST = W / SPA;  // staging threads per angle, pairs per thread
@@ -93,6 +100,8 @@ __global__ __launch_bounds__(64 * bp_waves(TY)) void bp_kernel(const BPTask* __r
   float yl[PY];
 #pragma unroll
   for (int r = 0; r < PY; ++r) yl[r] = (float)(wave * PY + r);
+  // the reference's jx - midpoint (1-based), exact in fp32
+  const float ux = (float)(x + 1) - m;
   // tile corners (1-based) for the staging windows
   const float cx0 = (float)(x0 + 1) - m, cx1 = (float)(x0 + TX) - m;
   const float cy0 = (float)(y0 + 1) - m, cy1 = (float)(y0 + TY) - m;
@@ -138,17 +147,41 @@ __global__ __launch_bounds__(64 * bp_waves(TY)) void bp_kernel(const BPTask* __r
     float blk[PY];
 #pragma unroll
     for (int r = 0; r < PY; ++r) blk[r] = 0.0f;
-    for (int a = 0; a < na; ++a) {
-      const float2 v = const_f2(sc, a0 + a);
-      const float base = fmaf(v.x, xl, k_s[a]);
-      const float2* Pa = P + a * W;
+    if (REFPOS) {
+      // the wave's rows of the row-term table: wave-uniform, scalar loads
+      const int ybu = __builtin_amdgcn_readfirstlane(yb);
+      const ACU_CONST float* rt = as_const(tk.rowterm) + (size_t)a0 * N + ybu;
+      for (int a = 0; a < na; ++a, rt += N) {
+#pragma clang fp contract(off)
+        const float sa = const_f2(sc, a0 + a).x;
+        const float A = sa * ux;  // sin(c)*(jx-midpoint), tv_hpc.F90:114
+        // byte offset of P[a*W - 1 - lo] (wave-uniform): the pair of the
+        // 1-based tap y1 is at Pb + 8*y1
+        const char* Pb = (const char*)P + __builtin_amdgcn_readfirstlane(8 * (a * W - 1 - lo_s[a]));
 #pragma unroll
-      for (int r = 0; r < PY; ++r) {
-        const float yr = fmaf(v.y, yl[r], base);  // detector position - 1 - lo, in (0, W - 1)
-        const int t = (int)yr;
-        const float fr = __builtin_amdgcn_fractf(yr);
-        const float2 pv = Pa[t];
-        blk[r] += fmaf(fr, pv.y, pv.x);
+        for (int r = 0; r < PY; ++r) {
+          const float yr = (A + rt[r]) + m;  // (sin(c)*(jx-m) + cos(c)*(jy-m)) + m
+          // y1 = floor(yr); truncation differs only for yr < 0, whose pairs
+          // (taps below 2) are staged as zeros either way
+          const int y1 = (int)yr;
+          const float fr = __builtin_amdgcn_fractf(yr);  // yr - floor(yr) = yr - y1, exact
+          const float2 pv = *(const float2*)(Pb + (y1 << 3));
+          blk[r] += fmaf(fr, pv.y, pv.x);
+        }
+      }
+    } else {
+      for (int a = 0; a < na; ++a) {
+        const float2 v = const_f2(sc, a0 + a);
+        const float base = fmaf(v.x, xl, k_s[a]);
+        const float2* Pa = P + a * W;
+#pragma unroll
+        for (int r = 0; r < PY; ++r) {
+          const float yr = fmaf(v.y, yl[r], base);  // detector position - 1 - lo, in (0, W - 1)
+          const int t = (int)yr;
+          const float fr = __builtin_amdgcn_fractf(yr);
+          const float2 pv = Pa[t];
+          blk[r] += fmaf(fr, pv.y, pv.x);
+        }
       }
     }
 #pragma unroll
@@ -170,17 +203,37 @@ __global__ __launch_bounds__(64 * bp_waves(TY)) void bp_kernel(const BPTask* __r
 
 }  // namespace
 
+__global__ __launch_bounds__(256) void rowterm_kernel(const float2* __restrict__ sc, int N, int T, float mid,
+                                                      float* __restrict__ out) {
+#pragma clang fp contract(off)
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (size_t)N * T + BP_ROWTERM_PAD) return;
+  const int t = (int)(i / N), y = (int)(i - (size_t)t * N);
+  out[i] = t < T ? sc[t].y * ((float)(y + 1) - mid) : 0.0f;
+}
+
+void launch_bp_rowterm(const float2* sc, int N, int T, float mid, float* rowterm, hipStream_t s) {
+  const size_t n = (size_t)N * T + BP_ROWTERM_PAD;
+  hipLaunchKernelGGL(rowterm_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sc, N, T, mid, rowterm);
+}
+
 int bp_tiles_x(int N) { return (N + TX - 1) / TX; }
 int bp_tiles_y(int N, int ty) { return (N + ty - 1) / ty; }
 
-void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, int ty, hipStream_t s) {
-  if (tiles <= 0) return;
+template <bool REFPOS>
+void launch_bp(const BPTask* d_tasks, int ntasks, int base, int tiles, int ty, hipStream_t s) {
   switch (ty) {
-    case 16: hipLaunchKernelGGL(bp_kernel<16>, dim3(tiles), dim3(64 * bp_waves(16)), 0, s, d_tasks, ntasks, base); break;
-    case 32: hipLaunchKernelGGL(bp_kernel<32>, dim3(tiles), dim3(64 * bp_waves(32)), 0, s, d_tasks, ntasks, base); break;
-    case 64: hipLaunchKernelGGL(bp_kernel<64>, dim3(tiles), dim3(64 * bp_waves(64)), 0, s, d_tasks, ntasks, base); break;
+    case 16: hipLaunchKernelGGL((bp_kernel<16, REFPOS>), dim3(tiles), dim3(64 * bp_waves(16)), 0, s, d_tasks, ntasks, base); break;
+    case 32: hipLaunchKernelGGL((bp_kernel<32, REFPOS>), dim3(tiles), dim3(64 * bp_waves(32)), 0, s, d_tasks, ntasks, base); break;
+    case 64: hipLaunchKernelGGL((bp_kernel<64, REFPOS>), dim3(tiles), dim3(64 * bp_waves(64)), 0, s, d_tasks, ntasks, base); break;
     default: throw Error("backprojection tile height must be 16, 32 or 64");
   }
+}
+
+void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, int ty, bool refpos, hipStream_t s) {
+  if (tiles <= 0) return;
+  if (refpos) launch_bp<true>(d_tasks, ntasks, base, tiles, ty, s);
+  else launch_bp<false>(d_tasks, ntasks, base, tiles, ty, s);
 }
 
 }  // namespace acu

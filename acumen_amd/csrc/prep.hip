@@ -35,6 +35,10 @@ __device__ __forceinline__ float median27(float (&v)[27]) {
 
 // 3x3x3 median (:420-447): projections l, c, r; neighbours clamped into the
 // image; the 14th of 27 in descending order = the 14th in ascending order.
+// This is the median the reference's loop nest describes: its register
+// slots 3*jxs+jys+2 (+9, +18) run from -2 to 24 in an array of 27 (:433-435
+// against :134), so its own code writes 3 neighbours out of bounds and sorts
+// 3 stale slots in their place; that undefined result is not reproduced.
 // Values that compare equal but differ in bits (+0 and -0) or NaN are the
 // exception: the reference's exchange sort is not stable, so which of them
 // it leaves in the middle depends on its order of exchanges.

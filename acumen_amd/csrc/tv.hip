@@ -75,7 +75,9 @@ __device__ __forceinline__ float lap(const float* __restrict__ u, int x, int y, 
 }
 
 // K9 + K10 + K11 (tv_hpc.F90:207-249): ukp = uk; rhs from d, b; r = rhs -
-// mu*iukt - lambda*L(uk); p = r; rs[0] = sum r^2.
+// mu*iukt - lambda*L(uk); p = r; rs[0] = sum r^2.  LIN: ifkt and iukt are
+// the linear schedule's FP64 carried terms, rounded to FP32 here.
+template <bool LIN>
 __global__ __launch_bounds__(NT_) void residual_kernel(const TVTask* __restrict__ tasks, int ntasks, int base) {
   const TVTask& tk = tasks[find_task_e(tasks, ntasks, base + blockIdx.x)];
   const int N = tk.N;
@@ -97,8 +99,10 @@ __global__ __launch_bounds__(NT_) void residual_kernel(const TVTask* __restrict_
       t = t - tk.bky[ey];
       t = t - tk.dky[e];
       t = t + tk.bky[e];
-      const float rhs = mu * tk.ifkt[e] + lam * t;
-      const float rv = (rhs - mu * tk.iukt[e]) - lam * lap(tk.uk, x, y, N);
+      const float ifkt = LIN ? (float)tk.ifkt64[e] : tk.ifkt[e];
+      const float iukt = LIN ? (float)tk.iukt64[e] : tk.iukt[e];
+      const float rhs = mu * ifkt + lam * t;
+      const float rv = (rhs - mu * iukt) - lam * lap(tk.uk, x, y, N);
       tk.r[e] = rv;
       tk.p[e] = rv;
       tk.ukp[e] = tk.uk[e];
@@ -133,13 +137,21 @@ __global__ __launch_bounds__(NT_) void apdot_kernel(const TVTask* __restrict__ t
 // uk += alpha p; r -= alpha Ap; rs[j+1] = sum r^2.
 // A zero p.Ap (all-zero residual) gives alpha = 0 instead of the
 // reference's 0/0 = NaN.
+// LIN (linear schedule): iukt = F(R(uk)) is carried along by linearity in
+// FP64, iukt64 += alpha irpt with irpt = F(R(p)), instead of being
+// recomputed from uk by a projection and a backprojection.
+__device__ __forceinline__ float cg_alpha(const double* scal, int j) {
+  const double num = scal[64 + j];
+  return num != 0.0 ? (float)(scal[j] / num) : 0.0f;
+}
+
+template <bool LIN>
 __global__ __launch_bounds__(NT_) void axpy_kernel(const TVTask* __restrict__ tasks, int ntasks, int base, int j) {
   const TVTask& tk = tasks[find_task_e(tasks, ntasks, base + blockIdx.x)];
   const int N = tk.N;
   const int lb = base + blockIdx.x - tk.eb_begin;
   const size_t NN = (size_t)N * N;
-  const double num = tk.scal[64 + j];
-  const float alpha = num != 0.0 ? (float)(tk.scal[j] / num) : 0.0f;
+  const float alpha = cg_alpha(tk.scal, j);
   double acc = 0.0;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
@@ -148,10 +160,27 @@ __global__ __launch_bounds__(NT_) void axpy_kernel(const TVTask* __restrict__ ta
       tk.uk[e] = tk.uk[e] + alpha * tk.p[e];
       const float rv = tk.r[e] - alpha * tk.Ap[e];
       tk.r[e] = rv;
+      if (LIN) tk.iukt64[e] += (double)alpha * (double)tk.irpt[e];
       acc += (double)rv * (double)rv;
     }
   }
   reduce_to_partial(acc, tk.partials, lb);
+}
+
+// Linear schedule, sinogram part of K14: ruk = R(uk) is carried along by
+// linearity in FP64, ruk64 += alpha rp with rp = R(p), so the add-back of
+// tv_hpc.F90:339-346 needs no projection of the new uk.
+__global__ __launch_bounds__(NT_) void sino_axpy_kernel(const TVTask* __restrict__ tasks, int ntasks, int base,
+                                                        int j) {
+  const TVTask& tk = tasks[find_task_s(tasks, ntasks, base + blockIdx.x)];
+  const int lb = base + blockIdx.x - tk.sb_begin;
+  const size_t NT = (size_t)tk.N * tk.T;
+  const float alpha = cg_alpha(tk.scal, j);
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
+    if (e < NT) tk.ruk64[e] += (double)alpha * (double)tk.rp[e];
+  }
 }
 
 // K16 (tv_hpc.F90:294-299): p = r + (rs[j+1]/rs[j]) p
@@ -205,6 +234,11 @@ __device__ __forceinline__ void shrink(float t, float inv_lambda, float& d, floa
 
 // K17 + K18 (tv_hpc.F90:305-333): anisotropic shrink of D u + b at 1/lambda,
 // Err_diff = sum |uk - ukp| (also appended to the per-slice delta log).
+// UPD (linear schedule, last inner iteration): the next outer iteration's
+// ifkt = F(fk + f - ruk) = ifkt + F(f) - F(ruk) = (ifkt + fbp) - iukt by
+// linearity of F (FP64 carried terms), instead of the backprojection of
+// tv_hpc.F90:215.
+template <bool UPD>
 __global__ __launch_bounds__(NT_) void shrink_kernel(const TVTask* __restrict__ tasks, int ntasks, int base) {
   const TVTask& tk = tasks[find_task_e(tasks, ntasks, base + blockIdx.x)];
   const int N = tk.N;
@@ -226,6 +260,7 @@ __global__ __launch_bounds__(NT_) void shrink_kernel(const TVTask* __restrict__ 
       shrink((tk.bky[e] + tk.uk[(size_t)yp * N + x]) - u, inv_lambda, d, b);
       tk.dky[e] = d;
       tk.bky[e] = b;
+      if (UPD) tk.ifkt64[e] = (tk.ifkt64[e] + (double)tk.fbp[e]) - tk.iukt64[e];
       acc += (double)fabsf(u - tk.ukp[e]);
     }
   }
@@ -233,6 +268,8 @@ __global__ __launch_bounds__(NT_) void shrink_kernel(const TVTask* __restrict__ 
 }
 
 // K19 (tv_hpc.F90:341-346): fk = (fk + f) - ruk, plus the delta log entry.
+// LIN: ruk is the linear schedule's FP64 carried term.
+template <bool LIN>
 __global__ __launch_bounds__(NT_) void addback_kernel(const TVTask* __restrict__ tasks, int ntasks, int base) {
   const TVTask& tk = tasks[find_task_s(tasks, ntasks, base + blockIdx.x)];
   const int lb = base + blockIdx.x - tk.sb_begin;
@@ -240,12 +277,41 @@ __global__ __launch_bounds__(NT_) void addback_kernel(const TVTask* __restrict__
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
-    if (e < NT) tk.fk[e] = (tk.fk[e] + tk.f[e]) - tk.ruk[e];
+    if (e < NT) {
+      if (LIN) tk.fk[e] = (float)(((double)tk.fk[e] + (double)tk.f[e]) - tk.ruk64[e]);
+      else tk.fk[e] = (tk.fk[e] + tk.f[e]) - tk.ruk[e];
+    }
   }
   if (lb == 0 && threadIdx.x == 0) {
     const int j = (int)tk.scal[126];
     if (tk.delta_log != nullptr && j < tk.delta_cap) tk.delta_log[j] = (float)tk.scal[127];
     tk.scal[126] = (double)(j + 1);
+  }
+}
+
+// Linear schedule: the FP64 carried terms from exact FP32 operator results
+// (image part: ifkt, iukt; sinogram part: ruk).
+__global__ __launch_bounds__(NT_) void linload_img_kernel(const TVTask* __restrict__ tasks, int ntasks, int base) {
+  const TVTask& tk = tasks[find_task_e(tasks, ntasks, base + blockIdx.x)];
+  const int lb = base + blockIdx.x - tk.eb_begin;
+  const size_t NN = (size_t)tk.N * tk.N;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
+    if (e < NN) {
+      tk.ifkt64[e] = (double)tk.ifkt[e];
+      tk.iukt64[e] = (double)tk.iukt[e];
+    }
+  }
+}
+__global__ __launch_bounds__(NT_) void linload_sino_kernel(const TVTask* __restrict__ tasks, int ntasks, int base) {
+  const TVTask& tk = tasks[find_task_s(tasks, ntasks, base + blockIdx.x)];
+  const int lb = base + blockIdx.x - tk.sb_begin;
+  const size_t NT = (size_t)tk.N * tk.T;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const size_t e = (size_t)lb * PER_BLOCK + it * NT_ + threadIdx.x;
+    if (e < NT) tk.ruk64[e] = (double)tk.ruk[e];
   }
 }
 
@@ -283,23 +349,34 @@ int ew_blocks(size_t n) { return (int)((n + PER_BLOCK - 1) / PER_BLOCK); }
 void launch_tv_finalize(const TVTask* t, int first, int count, int slot, hipStream_t s) {
   hipLaunchKernelGGL(finalize_kernel, dim3(count), dim3(NT_), 0, s, t, first, slot);
 }
-void launch_tv_residual(const TVTask* t, int n, int base, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base);
+void launch_tv_residual(const TVTask* t, int n, int base, int blocks, bool lin, hipStream_t s) {
+  if (lin) hipLaunchKernelGGL(residual_kernel<true>, dim3(blocks), dim3(NT_), 0, s, t, n, base);
+  else hipLaunchKernelGGL(residual_kernel<false>, dim3(blocks), dim3(NT_), 0, s, t, n, base);
 }
 void launch_tv_apdot(const TVTask* t, int n, int base, int blocks, int j, hipStream_t s) {
   hipLaunchKernelGGL(apdot_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base, j);
 }
-void launch_tv_axpy(const TVTask* t, int n, int base, int blocks, int j, hipStream_t s) {
-  hipLaunchKernelGGL(axpy_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base, j);
+void launch_tv_axpy(const TVTask* t, int n, int base, int blocks, int j, bool lin, hipStream_t s) {
+  if (lin) hipLaunchKernelGGL(axpy_kernel<true>, dim3(blocks), dim3(NT_), 0, s, t, n, base, j);
+  else hipLaunchKernelGGL(axpy_kernel<false>, dim3(blocks), dim3(NT_), 0, s, t, n, base, j);
+}
+void launch_tv_sino_axpy(const TVTask* t, int n, int base, int sblocks, int j, hipStream_t s) {
+  hipLaunchKernelGGL(sino_axpy_kernel, dim3(sblocks), dim3(NT_), 0, s, t, n, base, j);
 }
 void launch_tv_pupdate(const TVTask* t, int n, int base, int blocks, int j, hipStream_t s) {
   hipLaunchKernelGGL(pupdate_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base, j);
 }
-void launch_tv_shrink(const TVTask* t, int n, int base, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(shrink_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base);
+void launch_tv_shrink(const TVTask* t, int n, int base, int blocks, bool upd_ifkt, hipStream_t s) {
+  if (upd_ifkt) hipLaunchKernelGGL(shrink_kernel<true>, dim3(blocks), dim3(NT_), 0, s, t, n, base);
+  else hipLaunchKernelGGL(shrink_kernel<false>, dim3(blocks), dim3(NT_), 0, s, t, n, base);
 }
-void launch_tv_addback(const TVTask* t, int n, int base, int sblocks, hipStream_t s) {
-  hipLaunchKernelGGL(addback_kernel, dim3(sblocks), dim3(NT_), 0, s, t, n, base);
+void launch_tv_addback(const TVTask* t, int n, int base, int sblocks, bool lin, hipStream_t s) {
+  if (lin) hipLaunchKernelGGL(addback_kernel<true>, dim3(sblocks), dim3(NT_), 0, s, t, n, base);
+  else hipLaunchKernelGGL(addback_kernel<false>, dim3(sblocks), dim3(NT_), 0, s, t, n, base);
+}
+void launch_tv_linload(const TVTask* t, int n, int base, int blocks, int sbase, int sblocks, hipStream_t s) {
+  hipLaunchKernelGGL(linload_img_kernel, dim3(blocks), dim3(NT_), 0, s, t, n, base);
+  hipLaunchKernelGGL(linload_sino_kernel, dim3(sblocks), dim3(NT_), 0, s, t, n, sbase);
 }
 void launch_tv_init(const TVTask* t, int n, int base, int blocks, int sbase, int sblocks,
                     hipStream_t s) {

@@ -146,6 +146,22 @@ class Batch:
         check(self._lib.acu_batch_tvopt(self.handle, float(mu), float(lam), int(nouter),
                                         int(ninner), int(ncg), int(mode)))
 
+    def set_tv_schedule(self, sched: int, refresh: int = 0) -> None:
+        """Operator schedule of tvopt: _lib.ACU_TV_SCHED_LINEAR (default: 5
+        radon + 5 iradon per outer iteration), ACU_TV_SCHED_REUSE (6 + 7,
+        bit-identical to the reference's) or ACU_TV_SCHED_REFERENCE (7 + 7);
+        refresh > 0 recomputes the linear schedule's carried terms exactly
+        every `refresh` outer iterations."""
+        check(self._lib.acu_batch_set_tv_schedule(self.handle, int(sched), int(refresh)))
+
+    def tv_schedule(self) -> tuple[int, int]:
+        """(schedule, refresh period)."""
+        r = ctypes.c_int()
+        v = self._lib.acu_batch_tv_schedule(self.handle, ctypes.byref(r))
+        if v < 0:
+            check(-v)
+        return v, r.value
+
     # single-slice operators on host arrays (slice 0) -- the reference subroutines
     def radon(self, img) -> np.ndarray:
         N, T = self.shapes[0]

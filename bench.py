@@ -5,29 +5,37 @@ Workload (one slice): 2048 detector bins (odd-padded to N = 2049) x 1500
 angles on [0, 180), a 2048^2 phantom of 40 seeded ellipses, Poisson noise at
 I0 = 5e3, Ram-Lak FBP warm start then split-Bregman TV with mu = 0.05,
 lambda = 0.1, 50 outer x 1 inner x 5 CG iterations -- FP32 operators, FP64
-CG scalars.  A step is that reconstruction for a batch of `--batch` slices
-(default 32, one timed step of ~36 s) resident in HBM, run by the batched
-path (every kernel launch covers all slices, CG scalars stay on the device).
-`value` is slices/s over the timed steps.  Alongside it: the plain per-slice GPU path (one slice at a time, each
-reduction returned to the host before the next kernel, as the reference is
-structured) and FBP slices/s of the batched path.
+CG scalars.  A step is that whole reconstruction for a batch of `--batch`
+slices (default 4, ~3.5 s) resident in HBM, run by the batched path (every
+kernel launch covers all slices, CG scalars stay on the device).  `value`
+is slices/s over the timed steps, with HIP events bracketing every operator
+launch on the library's stream (they give the roofline's kernel times); the
+warm-up steps replay each outer iteration from a HIP graph instead and
+report their time beside it.  After the timed steps, each within what is
+left of --budget-s: FBP slices/s of the batched path; the plain per-slice
+GPU path (one slice at a time, each reduction returned to the host before
+the next kernel, as the reference is structured) with the same operators
+as the batched path (`batched_vs_plain`: batching alone) and with the
+reference's 7 radon + 7 iradon per outer iteration; the CPU baseline.
 
-roofline: the forward projector dominates TV time (6 of the 13 projector
-launches per outer iteration and ~74% of the time); its algorithmic work is
+roofline: the forward projector dominates TV time; its algorithmic work is
 14 FP32 flops per ray sample (2 coordinates, 2 fractions, 3 lerps, 1
-accumulate) over N*N*T samples per slice, measured with HIP events on the
-library's stream during the timed steps.  The kernel is bound by its LDS
-gathers (16 bytes of staged box per sample), so `roofline.lds` reports that
-rate as well.  `roofline_backprojection` gives the same figures for the
-backprojector (6 flop per pixel-angle update over N*N*T updates, compulsory
-bytes 4(N*T + 2*N*N) per slice) and its LDS read rate: 8 bytes per update
-over the updates of the tiles that hold mask pixels (the others only store
+accumulate) over N*N*T samples per slice.  It runs on the FP32 VALU with
+its bilinear gathers from LDS, so `bound` is "valu", `frac` is against the
+published FP32 vector peak (157.3 TFLOP/s, packed issue) and
+`frac_nonpacked` against the 78.6 TFLOP/s of one FMA per lane and clock,
+the rate of code built without packed FP32.  `traffic` is the HBM bytes of
+a launch from the rocprofv3 counter summary under profiles/.
+`roofline_backprojection` gives the same figures for the backprojector
+(6 flop per pixel-angle update over N*N*T updates, compulsory bytes
+4(N*T + 2*N*N) per slice) and its LDS read rate: 8 bytes per update over
+the updates of the tiles that hold mask pixels (the others only store
 zeros).
 
-cpu_baseline: the oracle's FP32 restatement of the reference's loops, one
-slice per host thread as the reference's MPI ranks do, on a bounded sample
-(one radon + one iradon per thread at 1/10 of the angles), extrapolated to a
-TV slice (350 radon + 351 iradon).
+cpu_baseline: the oracle's FP32 restatement of the reference's loops (the
+reference's own arithmetic, bit for bit) times tvopt on one slice of the
+workload with nouter = 0 and nouter = 1 on the host cores (at most 16);
+a slice costs t(0) + 50 (t(1) - t(0)).
 """
 from __future__ import annotations
 
@@ -47,9 +55,13 @@ MU, LAM, NOUTER, NINNER, NCG = 0.05, 0.1, 50, 1, 5
 FP_FLOPS_PER_SAMPLE = 14
 BP_FLOPS_PER_UPDATE = 6  # coordinate (2), fraction, lerp (2), accumulate
 GRAD_LAM, GRAD_EPS = 0.1, 1e-3  # differentiation workload: TV weight and smoothing
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3       # MI355X's published FP32 vector peak (packed FP32 issue)
+FP32_NONPACKED_TFLOPS = 78.6   # one FP32 FMA per lane and clock: the rate of non-packed code
 LDS_BYTES_PER_SAMPLE = 16  # the 2x2 cell of a bilinear sample
 N_CU, CLOCK_GHZ = 256, 2.4
+
+
+T_START = time.perf_counter()
 
 
 def log(*a):
@@ -111,26 +123,72 @@ def bp_active_fraction(N, wpu, ty):
     return float(tiles.sum()) * 64 * ty / (N * N)
 
 
-def cpu_baseline(N, T, wpu, threads):
-    """Oracle FP32 port: one slice per thread, one radon + one iradon at T/10
-    angles, extrapolated to a full TV slice.  Returns (slices/s, sample)."""
+def cpu_threads(requested: int = 0) -> int:
+    """Threads for a CPU leg: the host cores this process may run on, at most
+    16 (one GPU's share of a shared host), unless a count is requested."""
+    if requested > 0:
+        return requested
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_baseline_tv(ctx, sino, th, wpu, threads):
+    """CPU baseline of the headline: the oracle's FP32 restatement of the
+    reference's loops (bit for bit the reference's own arithmetic) runs tvopt
+    on one 2049^2 x 1500 slice with nouter = 0 (the FBP warm start) and
+    nouter = 1 (one complete outer iteration: 7 radon, 7 iradon and the
+    elementwise updates, tomography/tv_hpc.F90:205-353), its operators split
+    over `threads` host threads (each output element stays one sequential
+    sum).  A slice of the headline costs t(0) + NOUTER * (t(1) - t(0)): the
+    only extrapolation is in the outer-iteration count, in which the
+    reference's work is linear.  Also the checker: the GPU's FBP and one
+    outer iteration of the same slice against the oracle's results -- the
+    reference's FP32 arithmetic, and FP32 operators with FP64 CG scalars
+    (the GPU's precision policy: the reference's sequential FP32 dot
+    products over N*N pixels move its TV ~2e-4 from FP64 at N = 2049 after
+    one outer iteration, which FP64 scalars do not)."""
     import numpy as np
 
-    from acumen_amd import synth
+    from acumen_amd import ACU_IMG_FBP, ACU_IMG_TV, ACU_TV_PLAIN, Batch
     from oracle import ref
-    Ts = T // 10
-    th = synth.angles(Ts)
-    imgs = np.stack([synth.phantom(N, 40, 7000 + i) for i in range(threads)])
-    ref.build()
-    t0 = time.time()
-    ref.pass_batch_f32(imgs, th, wpu, threads)
-    wall = time.time() - t0
-    per_pass = wall * (T / Ts)                 # one radon + one iradon at T angles
-    per_slice = per_pass * (7 * NOUTER + 0.5)  # 350 radon + 351 iradon
-    sample = (f"{threads} slices in parallel, one per thread, each one radon + one iradon "
-              f"at {N}x{N} x {Ts} angles (1/10 of {T}): {wall:.1f} s wall; extrapolated "
-              f"x{T // Ts} in angles and x{7 * NOUTER + 0.5} passes per TV slice")
-    return threads / per_slice, sample, wall
+
+    N = sino.shape[1]
+    ref.set_threads(threads)
+    try:
+        t = time.perf_counter()
+        _, fbp0, _ = ref.tvopt(sino, th, wpu, MU, LAM, 0, NINNER, NCG, "f32")
+        t_fbp = time.perf_counter() - t
+        t = time.perf_counter()
+        tv1, _, _ = ref.tvopt(sino, th, wpu, MU, LAM, 1, NINNER, NCG, "f32")
+        t_one = time.perf_counter() - t
+        tv1d, _, _ = ref.tvopt(sino, th, wpu, MU, LAM, 1, NINNER, NCG, "f32d")
+    finally:
+        ref.set_threads(1)
+    per_slice = t_fbp + NOUTER * (t_one - t_fbp)
+    with Batch(ctx, [(N, sino.shape[0], wpu, th)]) as one:
+        one.set_sino(0, sino)
+        one.tvopt(MU, LAM, 1, NINNER, NCG, ACU_TV_PLAIN)
+        g_tv, g_fbp = one.get_image(0, ACU_IMG_TV), one.get_image(0, ACU_IMG_FBP)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64)))
+
+    try:
+        nproc = os.cpu_count()
+    except Exception:
+        nproc = None
+    return {"value": round(1.0 / per_slice, 6), "unit": "slices/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle FP32 tvopt of one {N}^2 x {sino.shape[0]} slice on {threads} threads "
+                       f"(os.cpu_count() = {nproc}): nouter=0 (FBP) {t_fbp:.1f} s, nouter=1 {t_one:.1f} s; "
+                       f"slice = t(0) + {NOUTER} x (t(1) - t(0)) = {per_slice:.0f} s"),
+            "fbp_s": round(t_fbp, 2), "outer_iteration_s": round(t_one - t_fbp, 2),
+            "gpu_vs_cpu_fbp_rel_l2": float(f"{rel(g_fbp, fbp0):.3e}"),
+            "gpu_vs_cpu_tv_1_outer_rel_l2": float(f"{rel(g_tv, tv1):.3e}"),
+            "gpu_vs_fp32_operators_fp64_scalars_tv_1_outer_rel_l2": float(f"{rel(g_tv, tv1d):.3e}"),
+            "cpu_fp32_vs_fp64_scalars_tv_1_outer_rel_l2": float(f"{rel(tv1, tv1d):.3e}")}
 
 
 def run_mixed(args) -> int:
@@ -313,7 +371,7 @@ def run_volume(args) -> int:
     cpu, acc = None, None
     if not args.no_cpu:
         log("cpu baseline: the oracle's FBP of 4 sampled slices; accuracy against its FP64 form")
-        cpu, acc = volume_cpu_baseline(sinos, th, wpu, samples, fbp, variants, args.cpu_threads)
+        cpu, acc = volume_cpu_baseline(sinos, th, wpu, samples, fbp, variants, cpu_threads(args.cpu_threads))
     ctx.close()
     out = {
         "metric": "FBP slices/s streamed (PCIe incl.), 2048^2 x 1500 x 256-slice volume; FBP+TV(30) slices/s",
@@ -660,7 +718,7 @@ def run_grad(args) -> int:
     cpu = None
     if not args.no_cpu:
         log("cpu baseline: the oracle's evaluation of one slice")
-        cpu = grad_cpu_baseline(xs[0], th, sh0[0], bs[0], args.cpu_threads)
+        cpu = grad_cpu_baseline(xs[0], th, sh0[0], bs[0], cpu_threads(args.cpu_threads))
     ctx.close()
     out = {
         "metric": "slice evaluations/s of the TV-regularised loss with gradients in the image and the "
@@ -690,16 +748,247 @@ def run_grad(args) -> int:
     return 0
 
 
+def run_production(args) -> int:
+    """The headline: split-Bregman TV of --batch resident 2049^2 x 1500
+    slices per step through the batched path, with per-launch HIP events on
+    the library's stream over the timed steps (they give the roofline
+    kernels' average launch durations).  After the timed steps, each within
+    what remains of --budget-s: FBP slices/s, the plain per-slice path with
+    the same operators (batching alone) and with the reference's operators,
+    and the CPU baseline."""
+    import numpy as np
+
+    from acumen_amd import ACU_IMG_FBP, ACU_IMG_TV, ACU_TV_BATCHED, ACU_TV_GRAPH, ACU_TV_PLAIN, Batch, Context, _lib, synth
+
+    N = synth.padded_size(N_DET)
+    T = T_ANG
+    B = args.batch
+    ctx = Context(0)
+    log(f"inputs: {B} slices of {N}x{N}, {T} angles")
+    th, wpu, sinos = make_inputs(ctx, B, N, T, 1001)
+    batch = Batch(ctx, [(N, T, wpu, th)] * B)
+    for i, f in enumerate(sinos):
+        batch.set_sino(i, f)
+    ctx.synchronize()
+    sched, refresh = batch.tv_schedule()
+    tuning = None
+    if not args.no_tune:
+        log("autotune: projector variant per slice shape, ramp-filter arithmetic, backprojection tile")
+        t = time.perf_counter()
+        rep = batch.autotune(reps=2)
+        tuning = {"seconds": round(time.perf_counter() - t, 2),
+                  "fp_variant": rep["slices"][0]["choice"], "ramp": rep["ramp"]["choice"],
+                  "bp_tile": rep["bp"]["choice"],
+                  "ramp_ms_per_batch": {k: round(v, 4) for k, v in rep["ramp"]["ms"].items()},
+                  "fp_ms_per_slice": {k: round(v, 4) for k, v in rep["slices"][0]["ms"].items()},
+                  "bp_ms_per_batch": {k: round(v, 4) for k, v in rep["bp"]["ms"].items()}}
+
+    # warm-up steps replay each outer iteration from a HIP graph (no events)
+    graph_ms = []
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_GRAPH)
+        ctx.synchronize()
+        graph_ms.append((time.perf_counter() - t) * 1e3)
+        log(f"warmup step {w + 1}/{args.warmup} (graph): {graph_ms[-1]:.0f} ms")
+
+    # timed steps: every launch bracketed by HIP events on the library's stream
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_BATCHED)
+        ctx.synchronize()
+        log(f"step {k + 1}/{args.steps} done at {time.perf_counter() - t0:.2f} s")
+    elapsed = time.perf_counter() - t0
+    kt = {name: ctx.kernel_time(kind) for name, kind in
+          (("ramp", _lib.KIND_RAMP), ("bp", _lib.KIND_BP), ("cells", _lib.KIND_CELLS),
+           ("fp", _lib.KIND_FP))}
+    ctx.set_timing(False)
+    tv_img = batch.get_image(0, ACU_IMG_TV)
+    assert np.all(np.isfinite(tv_img)), "non-finite TV result"
+    value = B * args.steps / elapsed
+    ms_per_step = elapsed / args.steps * 1e3
+
+    def remaining():
+        return args.budget_s - (time.perf_counter() - T_START)
+
+    legs = {}
+
+    def leg(name, cost_s, fn):
+        if remaining() < cost_s:
+            legs[name] = f"skipped: {remaining():.0f} s of the {args.budget_s:.0f} s budget left, needs ~{cost_s:.0f} s"
+            log(legs[name])
+            return None
+        try:
+            return fn()
+        except Exception as e:  # a supplementary leg never loses the headline
+            legs[name] = f"failed: {type(e).__name__}: {e}"
+            log(legs[name])
+            return None
+
+    # FBP slices/s of the batched path
+    def fbp_leg():
+        reps = 5
+        batch.fbp()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            batch.fbp()
+        ctx.synchronize()
+        return B * reps / (time.perf_counter() - t)
+
+    fbp_rate = leg("fbp", 5, fbp_leg)
+    per_slice_s = ms_per_step / 1e3 / B
+
+    # the plain per-slice path: one slice at a time, every reduction back on the host
+    def plain_leg(schedule):
+        with Batch(ctx, [(N, T, wpu, th)]) as one:
+            one.set_ramp_variant(batch.ramp_variant())
+            one.set_fp_variant(0, batch.fp_variant(0))
+            one.set_bp_tile(batch.bp_tile())
+            one.set_tv_schedule(schedule, refresh if schedule == sched else 0)
+            one.set_sino(0, sinos[0])
+            ctx.synchronize()
+            t = time.perf_counter()
+            one.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_PLAIN)
+            ctx.synchronize()
+            rate = 1.0 / (time.perf_counter() - t)
+            img = one.get_image(0, ACU_IMG_TV)
+        return rate, img
+
+    plain = None if args.no_plain else leg("plain", 4 * per_slice_s + 5, lambda: plain_leg(sched))
+    plain_rate = plain_equal = None
+    if plain is not None:
+        plain_rate = plain[0]
+        # full-size property: the batched path's slice 0 is the plain path's, bit for bit
+        plain_equal = bool(np.array_equal(plain[1].view(np.uint32), tv_img.view(np.uint32)))
+    ref_struct = None if args.no_plain else leg("plain_reference_operators", 5 * per_slice_s + 5,
+                                                lambda: plain_leg(_lib.ACU_TV_SCHED_REFERENCE))
+    ref_rate = ref_dist = None
+    if ref_struct is not None:
+        ref_rate = ref_struct[0]
+        ref_dist = float(np.linalg.norm(ref_struct[1].astype(np.float64) - tv_img)
+                         / np.linalg.norm(ref_struct[1].astype(np.float64)))
+
+    cpu = None
+    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        threads = cpu_threads(args.cpu_threads)
+        cpu = leg("cpu", 75, lambda: cpu_baseline_tv(ctx, sinos[0], th, wpu, threads))
+
+    batch.close()
+    ctx.close()
+
+    fp_ms, fp_n = kt["fp"]
+    fp_avg_ms = fp_ms / max(fp_n, 1)
+    samples_per_launch = float(N) * N * T * B
+    achieved = FP_FLOPS_PER_SAMPLE * samples_per_launch / (fp_avg_ms * 1e-3) / 1e12
+    fp_name = fp_kernel_name(tuning.get("fp_variant") if tuning else None)
+    traffic_file = f"{fp_name}_traffic_2049x1500.txt"
+    traffic = traffic_from_profile(traffic_file)
+    lds_bytes = LDS_BYTES_PER_SAMPLE * samples_per_launch / (fp_avg_ms * 1e-3)
+    roofline = {
+        "bound": "valu",
+        "kernel": f"{fp_name} (ray-driven bilinear projection on the FP32 VALU, gathers from LDS)",
+        "limiters": "VALU issue and LDS bank conflicts of the bilinear gathers (SQ counters under profiles/)",
+        "achieved": round(achieved, 3),
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+        "peak_nonpacked": FP32_NONPACKED_TFLOPS,
+        "frac_nonpacked": round(achieved / FP32_NONPACKED_TFLOPS, 4),
+        "traffic": None if traffic is None else traffic * B,
+        "traffic_source": None if traffic is None else
+        f"profiles/{traffic_file}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per 2049^2 x 1500 slice-launch, x {B}",
+        "traffic_over_algorithmic": None if traffic is None else
+        round(traffic / (4.0 * (N * N + N * T)), 1),
+        "avg_launch_ms": round(fp_avg_ms, 4),
+        "launches": fp_n,
+        "work_per_launch": f"{FP_FLOPS_PER_SAMPLE} flop x {N}*{N}*{T} samples x {B} slices",
+        "lds": {"bytes_per_sample": LDS_BYTES_PER_SAMPLE, "achieved_TBps": round(lds_bytes / 1e12, 2),
+                "B_per_clk_per_cu": round(lds_bytes / (N_CU * CLOCK_GHZ * 1e9), 1)},
+    }
+
+    bp_ms, bp_n = kt["bp"]
+    bp_avg_s = bp_ms / max(bp_n, 1) * 1e-3
+    bp_achieved = BP_FLOPS_PER_UPDATE * samples_per_launch / bp_avg_s / 1e12
+    bp_ty = tuning["bp_tile"] if tuning else 32
+    bp_active = bp_active_fraction(N, wpu, bp_ty)
+    bp_lds = 8.0 * samples_per_launch * bp_active / bp_avg_s
+    roofline_bp = {
+        "bound": "valu+lds",
+        "kernel": "bp_kernel (pixel-driven linear-interpolation backprojection on the FP32 VALU)",
+        "achieved": round(bp_achieved, 3),
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(bp_achieved / FP32_PEAK_TFLOPS, 4),
+        "frac_nonpacked": round(bp_achieved / FP32_NONPACKED_TFLOPS, 4),
+        "avg_launch_ms": round(bp_avg_s * 1e3, 4),
+        "launches": bp_n,
+        "work_per_launch": f"{BP_FLOPS_PER_UPDATE} flop x {N}*{N}*{T} updates x {B} slices",
+        "compulsory_hbm_GBps": round(4.0 * (N * T + 2.0 * N * N) * B / bp_avg_s / 1e9, 1),
+        # 8 bytes of staged sinogram pairs per update, over the tiles that hold mask pixels
+        "lds": {"bytes_per_update": 8, "tile_rows": bp_ty, "computed_fraction": round(bp_active, 4),
+                "achieved_TBps": round(bp_lds / 1e12, 2),
+                "B_per_clk_per_cu": round(bp_lds / (N_CU * CLOCK_GHZ * 1e9), 1)},
+    }
+    names = {_lib.ACU_TV_SCHED_REFERENCE: "reference (7 radon + 7 iradon per outer iteration)",
+             _lib.ACU_TV_SCHED_REUSE: "reuse (6 radon + 7 iradon)",
+             _lib.ACU_TV_SCHED_LINEAR: "linear (5 radon + 5 iradon)"}
+    ops = {_lib.ACU_TV_SCHED_REFERENCE: (7, 7), _lib.ACU_TV_SCHED_REUSE: (6, 7), _lib.ACU_TV_SCHED_LINEAR: (5, 5)}
+    out = {
+        "metric": "TV-SB slices/s batched vs plain @2048^2 (rel-L2<=1e-4 vs FP64); FBP slices/s",
+        "value": round(value, 5),
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"split-Bregman TV, {B} slices of {N_DET}^2 (N={N}) x {T} angles per step, "
+                               f"40-ellipse phantoms, I0=5e3, mu={MU}, lambda={LAM}, {args.nouter} outer x "
+                               f"{NCG} CG, FBP warm start, batched path, inputs resident in HBM",
+                   "slices_per_step": B, "nimage": N, "ntheta": T, "nouter": args.nouter,
+                   "ncg": NCG, "wpu": wpu, "operator_schedule": names[sched],
+                   "radon_iradon_per_outer": list(ops[sched]), "refresh": refresh,
+                   "timed_mode": "batched launches, HIP events around every operator launch"},
+        "graph_mode_ms_per_step": None if len(graph_ms) < 2 else round(float(np.median(graph_ms[1:])), 2),
+        "plain_slices_per_s": None if plain_rate is None else round(plain_rate, 5),
+        "batched_vs_plain": None if plain_rate is None else round(value / plain_rate, 3),
+        "tv_batched_equals_plain_bitwise": plain_equal,
+        "plain_reference_operators_slices_per_s": None if ref_rate is None else round(ref_rate, 5),
+        "batched_vs_plain_reference_operators": None if ref_rate is None else round(value / ref_rate, 3),
+        "tv_rel_l2_vs_reference_operators": None if ref_dist is None else float(f"{ref_dist:.3e}"),
+        "fbp_slices_per_s": None if fbp_rate is None else round(fbp_rate, 3),
+        "kernel_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
+        "roofline": roofline,
+        "roofline_backprojection": roofline_bp,
+        "cpu_baseline": cpu,
+        "autotune": tuning,
+        "legs": legs or None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default: production 1 step of 32 slices; ingest 50; others 2)")
+                    help="timed steps (default: production 5; ingest 50; others 2)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=32, help="slices per step (production workload)")
+    ap.add_argument("--batch", type=int, default=4, help="slices per step (production workload)")
+    ap.add_argument("--budget-s", type=float, default=400.0,
+                    help="production: wall-clock budget from start; the legs after the timed steps "
+                         "(FBP, plain paths, CPU) run only while it lasts (the timed work never shrinks)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-slice leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (default: the host cores this process may use, at most 16)")
     ap.add_argument("--nouter", type=int, default=NOUTER,
                     help="outer iterations (default: the workload's 50; fewer only for profiling runs)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
@@ -715,7 +1004,7 @@ def main() -> int:
                     help="volume workload: slices streamed through FBP + 30-iteration TV")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"ingest": 50, "production": 1}.get(args.workload, 2)
+        args.steps = {"ingest": 50, "production": 5}.get(args.workload, 2)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(slices are independent: several GPUs would each take a share of the slices)",
@@ -724,180 +1013,9 @@ def main() -> int:
     if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
         print("bench.py: multi-process launch is not supported (one GPU)", file=sys.stderr)
         return 2
-
-    if args.workload == "mixed":
-        return run_mixed(args)
-    if args.workload == "volume":
-        return run_volume(args)
-    if args.workload == "prep":
-        return run_prep(args)
-    if args.workload == "ingest":
-        return run_ingest(args)
-    if args.workload == "grad":
-        return run_grad(args)
-
-    import numpy as np
-
-    from acumen_amd import ACU_IMG_TV, ACU_TV_BATCHED, ACU_TV_PLAIN, Batch, Context, _lib, synth
-
-    N = synth.padded_size(N_DET)
-    T = T_ANG
-    B = args.batch
-    ctx = Context(0)
-    log(f"inputs: {B} slices of {N}x{N}, {T} angles")
-    th, wpu, sinos = make_inputs(ctx, B, N, T, 1001)
-    batch = Batch(ctx, [(N, T, wpu, th)] * B)
-    for i, f in enumerate(sinos):
-        batch.set_sino(i, f)
-    ctx.synchronize()
-    tuning = None
-    if not args.no_tune:
-        log("autotune: projector variant per slice shape, ramp-filter arithmetic, backprojection tile")
-        t = time.perf_counter()
-        rep = batch.autotune(reps=2)
-        tuning = {"seconds": round(time.perf_counter() - t, 2),
-                  "fp_variant": rep["slices"][0]["choice"], "ramp": rep["ramp"]["choice"],
-                  "bp_tile": rep["bp"]["choice"],
-                  "ramp_ms_per_batch": {k: round(v, 4) for k, v in rep["ramp"]["ms"].items()},
-                  "fp_ms_per_slice": {k: round(v, 4) for k, v in rep["slices"][0]["ms"].items()},
-                  "bp_ms_per_batch": {k: round(v, 4) for k, v in rep["bp"]["ms"].items()}}
-
-    for w in range(args.warmup):
-        log(f"warmup step {w + 1}/{args.warmup}")
-        batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_BATCHED)
-        ctx.synchronize()
-
-    ctx.set_timing(True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        batch.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_BATCHED)
-        ctx.synchronize()
-        log(f"step {k + 1}/{args.steps} done at {time.perf_counter() - t0:.2f} s")
-    elapsed = time.perf_counter() - t0
-    kt = {name: ctx.kernel_time(kind) for name, kind in
-          (("ramp", _lib.KIND_RAMP), ("bp", _lib.KIND_BP), ("cells", _lib.KIND_CELLS),
-           ("fp", _lib.KIND_FP))}
-    ctx.set_timing(False)
-    tv_img = batch.get_image(0, ACU_IMG_TV)
-    assert np.all(np.isfinite(tv_img)), "non-finite TV result"
-
-    slices = B * args.steps
-    value = slices / elapsed
-    ms_per_step = elapsed / args.steps * 1e3
-
-    # FBP slices/s of the batched path
-    reps = 5
-    batch.fbp()
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(reps):
-        batch.fbp()
-    ctx.synchronize()
-    fbp_rate = B * reps / (time.perf_counter() - t1)
-
-    plain_rate, plain_equal = None, None
-    if not args.no_plain:
-        log("plain per-slice path: 1 slice")
-        with Batch(ctx, [(N, T, wpu, th)]) as one:
-            one.set_ramp_variant(batch.ramp_variant())
-            one.set_fp_variant(0, batch.fp_variant(0))
-            one.set_bp_tile(batch.bp_tile())
-            one.set_sino(0, sinos[0])
-            ctx.synchronize()
-            t2 = time.perf_counter()
-            one.tvopt(MU, LAM, args.nouter, NINNER, NCG, ACU_TV_PLAIN)
-            ctx.synchronize()
-            plain_rate = 1.0 / (time.perf_counter() - t2)
-            # full-size property: the batched path's slice 0 is the plain path's, bit for bit
-            plain_equal = bool(np.array_equal(one.get_image(0, ACU_IMG_TV).view(np.uint32),
-                                              tv_img.view(np.uint32)))
-    batch.close()
-    ctx.close()
-
-    fp_ms, fp_n = kt["fp"]
-    fp_avg_ms = fp_ms / max(fp_n, 1)
-    samples_per_launch = float(N) * N * T * B
-    achieved = FP_FLOPS_PER_SAMPLE * samples_per_launch / (fp_avg_ms * 1e-3) / 1e12
-    fp_name = fp_kernel_name(tuning.get("fp_variant") if tuning else None)
-    traffic = traffic_from_profile(f"{fp_name}_traffic_2049x1500.txt")
-    lds_bytes = LDS_BYTES_PER_SAMPLE * samples_per_launch / (fp_avg_ms * 1e-3)
-    roofline = {
-        "bound": "mfma",
-        "kernel": f"{fp_name} (ray-driven bilinear projection on the FP32 VALU; compute roofline)",
-        "achieved": round(achieved, 3),
-        "peak": FP32_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-        "traffic": None if traffic is None else traffic * B,
-        "avg_launch_ms": round(fp_avg_ms, 4),
-        "launches": fp_n,
-        "work_per_launch": f"{FP_FLOPS_PER_SAMPLE} flop x {N}*{N}*{T} samples x {B} slices",
-        # the projector is bound by LDS gathers: 16 bytes of box per sample
-        "lds": {"bytes_per_sample": LDS_BYTES_PER_SAMPLE, "achieved_TBps": round(lds_bytes / 1e12, 2),
-                "B_per_clk_per_cu": round(lds_bytes / (N_CU * CLOCK_GHZ * 1e9), 1)},
-    }
-
-    bp_ms, bp_n = kt["bp"]
-    bp_avg_s = bp_ms / max(bp_n, 1) * 1e-3
-    bp_achieved = BP_FLOPS_PER_UPDATE * samples_per_launch / bp_avg_s / 1e12
-    bp_ty = tuning["bp_tile"] if tuning else 32
-    bp_active = bp_active_fraction(N, wpu, bp_ty)
-    bp_lds = 8.0 * samples_per_launch * bp_active / bp_avg_s
-    roofline_bp = {
-        "bound": "lds+valu",
-        "kernel": "bp_kernel (pixel-driven linear-interpolation backprojection on the FP32 VALU)",
-        "achieved": round(bp_achieved, 3),
-        "peak": FP32_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(bp_achieved / FP32_PEAK_TFLOPS, 4),
-        "avg_launch_ms": round(bp_avg_s * 1e3, 4),
-        "launches": bp_n,
-        "work_per_launch": f"{BP_FLOPS_PER_UPDATE} flop x {N}*{N}*{T} updates x {B} slices",
-        "compulsory_hbm_GBps": round(4.0 * (N * T + 2.0 * N * N) * B / bp_avg_s / 1e9, 1),
-        # 8 bytes of staged sinogram pairs per update, over the tiles that hold mask pixels
-        "lds": {"bytes_per_update": 8, "tile_rows": bp_ty, "computed_fraction": round(bp_active, 4),
-                "achieved_TBps": round(bp_lds / 1e12, 2),
-                "B_per_clk_per_cu": round(bp_lds / (N_CU * CLOCK_GHZ * 1e9), 1)},
-    }
-
-    cpu = None
-    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
-        log(f"cpu baseline: {args.cpu_threads} threads")
-        rate, sample, wall = cpu_baseline(N, T, wpu, args.cpu_threads)
-        cpu = {"value": rate, "unit": "slices/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": sample}
-
-    out = {
-        "metric": "TV-SB slices/s batched vs plain @2048^2 (rel-L2<=1e-4 vs FP64); FBP slices/s",
-        "value": round(value, 5),
-        "unit": "slices/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 2),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"split-Bregman TV, {B} slices of {N_DET}^2 (N={N}) x {T} angles, "
-                               f"I0=5e3, mu={MU}, lambda={LAM}, {args.nouter} outer x {NCG} CG, "
-                               "FBP warm start, batched path",
-                   "slices_per_step": B, "nimage": N, "ntheta": T, "nouter": args.nouter,
-                   "ncg": NCG, "wpu": wpu},
-        "plain_slices_per_s": None if plain_rate is None else round(plain_rate, 5),
-        "batched_vs_plain": None if plain_rate is None else round(value / plain_rate, 3),
-        "tv_batched_equals_plain_bitwise": plain_equal,
-        "fbp_slices_per_s": round(fbp_rate, 3),
-        "kernel_avg_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
-        "roofline": roofline,
-        "roofline_backprojection": roofline_bp,
-        "cpu_baseline": cpu,
-        "autotune": tuning,
-    }
-    print(json.dumps(out), flush=True)
-    return 0
+    runners = {"mixed": run_mixed, "volume": run_volume, "prep": run_prep, "ingest": run_ingest,
+               "grad": run_grad, "production": run_production}
+    return runners[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -57,6 +57,12 @@ extern "C" {
 #define ACU_TV_BATCHED 1  /* all slices in every launch, scalars stay on the device */
 #define ACU_TV_GRAPH 2    /* ACU_TV_BATCHED with each outer iteration replayed from a HIP graph */
 
+/* Operator schedule of tvopt's outer iteration (acu_batch_set_tv_schedule);
+ * with ninner = 1, per outer iteration: */
+#define ACU_TV_SCHED_REFERENCE 0 /* 7 radon + 7 iradon, as tv_hpc.F90:205-346 */
+#define ACU_TV_SCHED_REUSE 1     /* 6 + 7: the radon of :228 is the add-back's radon(uk) of :339 (same bits) */
+#define ACU_TV_SCHED_LINEAR 2    /* 5 + 5 (default): radon(uk), iradon(radon(uk)) and iradon(fk) carried by linearity */
+
 typedef struct acu_ctx acu_ctx;
 typedef struct acu_batch acu_batch;
 
@@ -89,11 +95,14 @@ int acu_batch_destroy(acu_batch *b);
 
 int acu_batch_nslices(const acu_batch *b);
 
-/* theta(1:ntheta) in degrees for one slice (tv_hpc.F90:560-562). */
+/* theta(1:ntheta) in degrees for one slice (tv_hpc.F90:560-562).  The
+ * slice's FBP and TV images of the old angles are invalidated
+ * (acu_batch_get_image then returns ACU_ERR_STATE). */
 int acu_batch_set_theta(acu_batch *b, int slice, const float *theta_deg);
 
 /* Module array f(N,T) of one slice from host memory (tv_hpc.F90:614-618 +
- * `update device(f)` at :172). */
+ * `update device(f)` at :172).  The slice's FBP and TV images of the old
+ * sinogram are invalidated. */
 int acu_batch_set_sino(acu_batch *b, int slice, const float *f_host);
 
 /* Copy an image (ACU_IMG_TV or ACU_IMG_FBP, N*N floats) of one slice to host
@@ -117,8 +126,29 @@ int acu_batch_fbp(acu_batch *b);
 int acu_batch_tvopt(acu_batch *b, float mu, float lambda, int nouter, int ninner, int ncg,
                     int mode);
 
+/* Operator schedule of the batch's tvopt (ACU_TV_SCHED_*), any mode.  The
+ * reference (tv_hpc.F90:205-346) recomputes, every outer iteration,
+ * ifkt = iradon(fk) (:215), ruk = radon(uk) and iukt = iradon(ruk)
+ * (:228-229) and radon(uk) again for the add-back (:339).  radon and iradon
+ * are linear and the CG already forms rp = radon(p) and irpt = iradon(rp),
+ * so ACU_TV_SCHED_LINEAR carries ruk += alpha rp, iukt += alpha irpt and,
+ * after the add-back fk = fk + f - ruk, ifkt = ifkt + fbp - iukt (fbp =
+ * iradon(f)), from exact values formed in the prologue: 5 radon + 5 iradon
+ * per outer iteration instead of 7 + 7, equal in exact arithmetic and within
+ * FP32 rounding in practice.  refresh > 0 recomputes the three exactly every
+ * `refresh` outer iterations (0: never).  ACU_TV_SCHED_REUSE skips only the
+ * radon of :228 (bit-identical to the reference schedule).  Defaults come
+ * from ACUMEN_TV_SCHEDULE (linear | reuse | reference) and ACUMEN_TV_REFRESH,
+ * else LINEAR with no refresh.  No counterpart in the reference. */
+int acu_batch_set_tv_schedule(acu_batch *b, int sched, int refresh);
+/* The schedule (ACU_TV_SCHED_*, or -ACU_ERR_ARG); *refresh (may be NULL)
+ * receives the refresh period. */
+int acu_batch_tv_schedule(const acu_batch *b, int *refresh);
+
 /* Err_diff (the reference's "delta", tv_hpc.F90:327-333,350-352) after each
- * outer iteration of the last tvopt, n <= nouter values. */
+ * outer iteration of the last tvopt, n <= that tvopt's nouter values.
+ * ACU_ERR_STATE when no tvopt has completed on the slice since its sinogram
+ * or angles were last set. */
 int acu_batch_get_delta(acu_batch *b, int slice, float *delta_host, int n);
 
 /* radon(sinogram, img) (tv_hpc.F90:129-159) on device memory of slice

@@ -20,7 +20,18 @@ static inline float clampf(float v, float lo, float hi) { return v < lo ? lo : v
 
 /* 3x3x3 median of a projection and its two neighbours, clamped at the image
  * edges (projections_to_sinogram_hpc.F90:420-447): the 14th of the 27 values
- * in descending order. */
+ * in descending order.
+ *
+ * This is the median the reference intends, not the one its code computes.
+ * It stores neighbour (jxs, jys) of datal at register slot
+ * 3*(jxs+1-1)+jys+2 = 3*jxs+jys+2 (:433), which runs from -2 to 6, of datac
+ * at +9 (7..15) and of datar at +18 (16..24), in an array allocated
+ * register(nimage,nimage,27) (:134).  So the jxs = -1 column of datal is
+ * written out of bounds (slots -2..0, before the array), and slots 25-27 are
+ * never written: the sort (:438-447) then ranks 24 neighbours with 3 values
+ * left over from the previous projection at that pixel (or never set, at the
+ * first).  That result depends on undefined memory, so the oracle and the
+ * GPU kernel take the 27-neighbour median the loop nest describes. */
 void orc_prep_median27(int n, const float *l, const float *c, const float *r, float *out)
 {
     for (int jy = 1; jy <= n; ++jy)

@@ -23,6 +23,11 @@ def blob_image(n, seed):
 
 @pytest.mark.parametrize("n", [9, 16, 31])
 def test_median27_is_numpy_median(n):
+    """The pinned median is the intended 27-neighbour one.  The reference's
+    code stores its register slots at 3*jxs+jys+2 (+9, +18), which run from
+    -2 to 24 in an array of 27 (projections_to_sinogram_hpc.F90:433-435
+    against :134): 3 neighbours go out of bounds and 3 stale slots are sorted
+    in their place, an undefined result the oracle does not reproduce."""
     rng = np.random.default_rng(n)
     l, c, r = [rng.integers(0, 5, size=(n, n)).astype(np.float32) for _ in range(3)]  # many ties
     got = ref.prep_median27(l, c, r)
