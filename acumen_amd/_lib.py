@@ -85,6 +85,7 @@ SIGNATURES = {
                                       _vp]),
     "acu_prep_mirror_error_dev": (_c_int, [_vp, _c_int, _c_int, _c_float, _c_float, _vp, _vp,
                                            ctypes.POINTER(ctypes.c_double)]),
+    "acu_prep_mirror_error_noalloc": (_c_int, [_vp, _c_int, _c_int, _c_float, _c_float, _vp, _vp, _vp]),
     "acu_prep_find_center": (_c_int, [_vp, _c_int, _fp, _fp, _fp, _fp, ctypes.POINTER(_c_int)]),
     "acu_prep_sinograms": (_c_int, [_vp, _c_int, _c_int, _fp, _c_float, _c_float, _fp]),
     "acu_ingest_mean_dev": (_c_int, [_vp, _c_int, ctypes.c_longlong, _vp, _vp]),

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["api.hip", "filter.hip", "fft.hip", "bp.hip", "fp.hip", "fpl.hip", "fplc.hip", "tv.hip", "grad.hip", "prep.hip"]
+SOURCES = ["api.hip", "filter.hip", "fft.hip", "bp.hip", "fp.hip", "fpl.hip", "fplc.hip", "tv.hip", "grad.hip", "prep.hip", "noalloc.hip"]
 LIB = os.path.join(HERE, "libacumen_hip.so")
 ARCH = os.environ.get("ACUMEN_OFFLOAD_ARCH", "gfx950")
 

@@ -255,4 +255,7 @@ int ctx_device(const acu_ctx* c);
 hipStream_t ctx_stream(const acu_ctx* c);
 // Record msg as the calling thread's acu_last_error text; returns code.
 int record_error(int code, const std::string& msg);
+// Device address of the no-alloc entry point's static scratch on `device`
+// (noalloc.hip), looked up once per device with the device current.
+void noalloc_setup(int device);
 }  // namespace acu

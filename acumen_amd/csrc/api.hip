@@ -1280,7 +1280,9 @@ int acu_ctx_create(int device, acu_ctx** out) {
   try {
     ACU_HIP_CHECK(hipSetDevice(device));
     ACU_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    acu::noalloc_setup(device);  // the device address of this GPU's no-alloc scratch
   } catch (const Error& e) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(ACU_ERR_HIP, e.msg);
   }

@@ -306,6 +306,21 @@ int acu_prep_correct_dev(acu_ctx *ctx, int n, int add_nx, float theta, float shi
  * Synchronous. */
 int acu_prep_mirror_error_dev(acu_ctx *ctx, int n, int add_nx, float theta, float shift,
                               const float *p0_dev, const float *p180_dev, double *err);
+/* No counterpart in the reference: the library's one entry point that
+ * allocates nothing, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * mirror error of acu_prep_mirror_error_dev with its FP64 block partials in
+ * a static device buffer of ACU_NOALLOC_SCRATCH_DOUBLES doubles (512 KiB per
+ * GPU, held until the process ends) instead of memory allocated per call,
+ * and the result written to device memory err_dev (one double), both on the
+ * context's stream with no wait: read err_dev after acu_ctx_synchronize or
+ * a later stream-ordered copy.  Same bits as acu_prep_mirror_error_dev.
+ * ACU_ERR_ARG when ceil((n + add_nx) * n / 256) > ACU_NOALLOC_SCRATCH_DOUBLES
+ * (n at most 4095).  Every call on one GPU shares that GPU's buffer: calls
+ * that overlap on one GPU (other streams or threads) race. */
+#define ACU_NOALLOC_SCRATCH_DOUBLES 65536
+int acu_prep_mirror_error_noalloc(acu_ctx *ctx, int n, int add_nx, float theta, float shift,
+                                  const float *p0_dev, const float *p180_dev, double *err_dev);
 /* Centre-of-rotation and tilt search (:231-364): centroid estimate, then the
  * reference's pattern search on (theta, shift) with every trial's mirror
  * error on the GPU.  Host projections in; theta, shift and (may be NULL)
