@@ -28,6 +28,9 @@ ACU_TV_SCHED_REFERENCE = 0
 ACU_TV_SCHED_REUSE = 1
 ACU_TV_SCHED_LINEAR = 2
 
+ACU_BP_POS_REFERENCE = 0
+ACU_BP_POS_TILE = 1
+
 KIND_RAMP, KIND_BP, KIND_CELLS, KIND_FP, KIND_ADJ, KIND_PREP, KIND_INGEST = 0, 1, 2, 3, 4, 5, 6
 RAMP_FP64, RAMP_FP32, RAMP_BF16X3, RAMP_BF16X6, RAMP_FFT = 0, 1, 2, 3, 4
 
@@ -54,6 +57,8 @@ SIGNATURES = {
     "acu_batch_get_delta": (_c_int, [_vp, _c_int, _fp, _c_int]),
     "acu_batch_set_tv_schedule": (_c_int, [_vp, _c_int, _c_int]),
     "acu_batch_tv_schedule": (_c_int, [_vp, _ip]),
+    "acu_batch_set_bp_positions": (_c_int, [_vp, _c_int, _c_int]),
+    "acu_batch_bp_positions": (_c_int, [_vp, _ip, _ip]),
     "acu_radon_dev": (_c_int, [_vp, _c_int, _vp, _vp]),
     "acu_iradon_dev": (_c_int, [_vp, _c_int, _vp, _vp]),
     "acu_ramp_dev": (_c_int, [_vp, _c_int, _vp, _vp]),

@@ -192,6 +192,7 @@ struct Slice {
   float *uk = nullptr, *ukp = nullptr, *dkx = nullptr, *dky = nullptr, *bkx = nullptr,
         *bky = nullptr, *ifkt = nullptr, *iukt = nullptr, *r = nullptr, *p = nullptr,
         *irpt = nullptr, *Ap = nullptr, *fk = nullptr;
+  float* ftv = nullptr;  // iradon(f) with tvopt's backprojection positions, when they differ from FBP's
   double *ifkt64 = nullptr, *iukt64 = nullptr, *ruk64 = nullptr;
   double *scal = nullptr, *partials = nullptr;
   float* delta = nullptr;
@@ -253,10 +254,11 @@ struct acu_batch {
   int fl_total[acu::FL_GROUPS] = {0, 0, 0, 0};  // LDS projector blocks per variant group (lds1, lds2, lds4, cell16)
   int ramp_variant = acu::RAMP_FP64;  // arithmetic of the ramp filter
   int bp_ty = acu::BP_TY_DEFAULT;     // backprojection tile height (16, 32 or 64)
-  bool bp_refpos = true;              // backprojection positions in the reference's arithmetic
+  bool bp_refpos = true;              // FBP backprojection positions in the reference's arithmetic
+  bool tv_bp_refpos = true;           // the same inside tvopt's iteration
   // task sets for the fixed operand combinations of FBP / TV
   acu::RFSet rf_f, rf_fk, rf_ruk, rf_rp;  // ramp filter of f, fk, ruk, rp
-  acu::DevArray<acu::BPTask> bp_fbp, bp_ifkt, bp_iukt, bp_irpt;
+  acu::DevArray<acu::BPTask> bp_fbp, bp_ftv, bp_ifkt, bp_iukt, bp_irpt;
   acu::FPSet fp_uk, fp_p;  // radon(uk) -> ruk, radon(p) -> rp
   acu::DevArray<acu::TVTask> tv;
   float tv_mu = 0.f, tv_lambda = 0.f;
@@ -289,7 +291,7 @@ struct acu_batch {
   ~acu_batch() {
     drop_graphs();
     rf_f.release(); rf_fk.release(); rf_ruk.release(); rf_rp.release();
-    bp_fbp.release(); bp_ifkt.release(); bp_iukt.release(); bp_irpt.release();
+    bp_fbp.release(); bp_ftv.release(); bp_ifkt.release(); bp_iukt.release(); bp_irpt.release();
     fp_uk.release(); fp_p.release(); tv.release();
     fp_g.release(); res_g.release(); adj_g.release();
     tvg_g.release(); loss_g.release();
@@ -349,6 +351,7 @@ void layout_tv(acu_batch* b, Carver& cv, int delta_cap) {
     float** imgs[] = {&s.uk, &s.ukp, &s.dkx, &s.dky, &s.bkx, &s.bky,
                       &s.ifkt, &s.iukt, &s.r, &s.p, &s.irpt, &s.Ap};
     for (float** a : imgs) *a = cv.take<float>(NN);
+    s.ftv = cv.take<float>(NN);
     s.fk = cv.take<float>(NT);
     // the linear schedule's carried terms, FP64
     s.ifkt64 = cv.take<double>(NN);
@@ -567,12 +570,17 @@ void build_tv_tasks(acu_batch* b) {
   build_rfset(b, b->rf_fk, [](Slice& s) { return (const float*)s.fk; }, [](Slice& s) { return s.sinoCS; });
   build_rfset(b, b->rf_ruk, [](Slice& s) { return (const float*)s.ruk; }, [](Slice& s) { return s.sinoCS; });
   build_rfset(b, b->rf_rp, [](Slice& s) { return (const float*)s.rp; }, [](Slice& s) { return s.sinoCS; });
+  build(b, b->bp_ftv, [](Slice& s) { return bp_task(s, s.sinoCS, s.ftv); });
   build(b, b->bp_ifkt, [](Slice& s) { return bp_task(s, s.sinoCS, s.ifkt); });
   build(b, b->bp_iukt, [](Slice& s) { return bp_task(s, s.sinoCS, s.iukt); });
   build(b, b->bp_irpt, [](Slice& s) { return bp_task(s, s.sinoCS, s.irpt); });
   build_fpset(b, b->fp_uk, [](Slice& s) { return (const float*)s.uk; }, [](Slice& s) { return s.ruk; });
   build_fpset(b, b->fp_p, [](Slice& s) { return (const float*)s.p; }, [](Slice& s) { return s.rp; });
 }
+
+// tvopt's backprojection positions differ from the FBP's: its iteration
+// carries iradon(f) with its own positions (s.ftv) instead of the FBP image
+bool tv_own_fbp(const acu_batch* b) { return b->tv_bp_refpos != b->bp_refpos; }
 
 void build_tv_task_array(acu_batch* b) {
   build(b, b->tv, [b](Slice& s) {
@@ -583,7 +591,7 @@ void build_tv_task_array(acu_batch* b) {
     t.mu = b->tv_mu; t.lambda = b->tv_lambda;
     t.uk = s.uk; t.ukp = s.ukp; t.dkx = s.dkx; t.dky = s.dky; t.bkx = s.bkx; t.bky = s.bky;
     t.ifkt = s.ifkt; t.iukt = s.iukt; t.r = s.r; t.p = s.p; t.irpt = s.irpt; t.Ap = s.Ap;
-    t.f = s.f; t.fk = s.fk; t.ruk = s.ruk; t.rp = s.rp; t.fbp = s.fbp;
+    t.f = s.f; t.fk = s.fk; t.ruk = s.ruk; t.rp = s.rp; t.fbp = tv_own_fbp(b) ? s.ftv : s.fbp;
     t.ifkt64 = s.ifkt64; t.iukt64 = s.iukt64; t.ruk64 = s.ruk64;
     t.scal = s.scal; t.partials = s.partials;
     t.delta_log = s.delta; t.delta_cap = s.delta_cap;
@@ -631,11 +639,13 @@ void ramp_seq(acu_batch* b, const RFSet& rf, int sl) {
   }
 }
 
-void iradon_seq(acu_batch* b, const RFSet& rf, const DevArray<BPTask>& bp, int sl) {
+// refpos: the backprojection positions (the batch's FBP positions, or
+// tvopt's inside its iteration)
+void iradon_seq(acu_batch* b, const RFSet& rf, const DevArray<BPTask>& bp, int sl, bool refpos) {
   acu_ctx* c = b->ctx;
   const Range r2 = bp_range(b, sl);
   ramp_seq(b, rf, sl);
-  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, b->bp_ty, b->bp_refpos, c->stream); });
+  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, b->bp_ty, refpos, c->stream); });
   check_launch();
 }
 
@@ -711,10 +721,11 @@ void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool ruk_fresh, bool 
   const bool lin = b->tv_sched == ACU_TV_SCHED_LINEAR;
   for (int ji = 0; ji < ninner; ++ji) {
     const bool exact = !lin || (refresh && ji == 0);
-    if (exact) iradon_seq(b, b->rf_fk, b->bp_ifkt, sl);            // :215
+    const bool tp = b->tv_bp_refpos;
+    if (exact) iradon_seq(b, b->rf_fk, b->bp_ifkt, sl, tp);        // :215
     const bool reuse = b->tv_sched == ACU_TV_SCHED_REUSE && ruk_fresh && ji == 0;
     if (exact && !reuse) radon_seq(b, b->fp_uk, sl);               // :228
-    if (exact) iradon_seq(b, b->rf_ruk, b->bp_iukt, sl);           // :229
+    if (exact) iradon_seq(b, b->rf_ruk, b->bp_iukt, sl, tp);       // :229
     if (lin && exact) launch_tv_linload(tv, nt, ew.base, ew.n, sw.base, sw.n, c->stream);
     launch_tv_residual(tv, nt, ew.base, ew.n, lin, c->stream);     // :207-249
     launch_tv_finalize(tv, first, count, 0, c->stream);
@@ -722,7 +733,7 @@ void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool ruk_fresh, bool 
     if (host_sync) (void)read_scalar(b, sl, 0);
     for (int j = 0; j < ncg; ++j) {                                // :252-303
       radon_seq(b, b->fp_p, sl);
-      iradon_seq(b, b->rf_rp, b->bp_irpt, sl);
+      iradon_seq(b, b->rf_rp, b->bp_irpt, sl, tp);
       launch_tv_apdot(tv, nt, ew.base, ew.n, j, c->stream);
       launch_tv_finalize(tv, first, count, 64 + j, c->stream);
       check_launch();
@@ -746,30 +757,43 @@ void outer_body(acu_batch* b, int sl, int ninner, int ncg, bool ruk_fresh, bool 
 }
 
 void fbp_all(acu_batch* b, int sl) {
-  iradon_seq(b, b->rf_f, b->bp_fbp, sl);
+  iradon_seq(b, b->rf_f, b->bp_fbp, sl, b->bp_refpos);
+}
+
+// the backprojection alone (its ramp-filtered input is already in place)
+void bp_seq(acu_batch* b, const DevArray<BPTask>& bp, int sl, bool refpos) {
+  acu_ctx* c = b->ctx;
+  const Range r2 = bp_range(b, sl);
+  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, b->bp_ty, refpos, c->stream); });
+  check_launch();
 }
 
 // Prologue of tvopt (tv_hpc.F90:176-202) for all slices or one: FBP warm
 // start, uk = fbp, d = b = 0, fk = f; then what the schedule carries:
-// ruk = radon(uk) (REUSE, LINEAR), iukt = iradon(ruk) and ifkt = iradon(f),
-// which is the FBP itself (LINEAR).
+// ruk = radon(uk) (REUSE, LINEAR), iukt = iradon(ruk) and ifkt = iradon(f)
+// (LINEAR): the FBP itself, or, when tvopt's backprojection positions differ
+// from the FBP's, the same ramp-filtered f backprojected with tvopt's
+// positions into s.ftv.
 void tv_prologue(acu_batch* b, int sl, int nouter) {
   acu_ctx* c = b->ctx;
   fbp_all(b, sl);
   const int i0 = sl < 0 ? 0 : sl, i1 = sl < 0 ? (int)b->s.size() : sl + 1;
   const bool lin = b->tv_sched == ACU_TV_SCHED_LINEAR;
+  const bool own = lin && nouter > 0 && tv_own_fbp(b);
+  if (own) bp_seq(b, b->bp_ftv, sl, b->tv_bp_refpos);  // sinoCS still holds the ramp of f
   for (int i = i0; i < i1; ++i) {
     const Slice& s = b->s[i];
     const size_t bytes = (size_t)s.N * s.N * 4;
     ACU_HIP_CHECK(hipMemcpyAsync(s.uk, s.fbp, bytes, hipMemcpyDeviceToDevice, c->stream));
-    if (lin && nouter > 0) ACU_HIP_CHECK(hipMemcpyAsync(s.ifkt, s.fbp, bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (lin && nouter > 0)
+      ACU_HIP_CHECK(hipMemcpyAsync(s.ifkt, own ? s.ftv : s.fbp, bytes, hipMemcpyDeviceToDevice, c->stream));
   }
   const Range ew = ew_range(b, sl), sw = sw_range(b, sl);
   launch_tv_init(b->tv.d, b->tv.n, ew.base, ew.n, sw.base, sw.n, c->stream);
   check_launch();
   if (nouter > 0 && b->tv_sched != ACU_TV_SCHED_REFERENCE) radon_seq(b, b->fp_uk, sl);
   if (nouter > 0 && lin) {
-    iradon_seq(b, b->rf_ruk, b->bp_iukt, sl);
+    iradon_seq(b, b->rf_ruk, b->bp_iukt, sl, b->tv_bp_refpos);
     launch_tv_linload(b->tv.d, b->tv.n, ew.base, ew.n, sw.base, sw.n, c->stream);
     check_launch();
   }
@@ -1350,8 +1374,11 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
       b->ramp_variant = r == "fp32" ? acu::RAMP_FP32 : r == "bf16x3" ? acu::RAMP_BF16X3
                         : r == "bf16x6" ? acu::RAMP_BF16X6 : r == "fft" ? acu::RAMP_FFT : acu::RAMP_FP64;
     }
-    // backprojection positions: ACUMEN_BP_POSITIONS = reference (default) | tile
+    // backprojection positions: ACUMEN_BP_POSITIONS = reference (default) |
+    // tile for FBP images, ACUMEN_TV_BP_POSITIONS = reference (default) |
+    // tile inside tvopt's iteration
     if (const char* pv2 = std::getenv("ACUMEN_BP_POSITIONS")) b->bp_refpos = std::strcmp(pv2, "tile") != 0;
+    if (const char* pv3 = std::getenv("ACUMEN_TV_BP_POSITIONS")) b->tv_bp_refpos = std::strcmp(pv3, "tile") != 0;
     // backprojection tile height: ACUMEN_BP_TILE = 16 | 32 (default) | 64
     if (const char* tv = std::getenv("ACUMEN_BP_TILE")) {
       const int ty = std::atoi(tv);
@@ -1519,6 +1546,23 @@ int acu_batch_set_tv_schedule(acu_batch* b, int sched, int refresh) {
   if (refresh < 0) return fail(ACU_ERR_ARG, "acu_batch_set_tv_schedule: refresh must be >= 0");
   ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync(); b->tv_sched = sched; b->tv_refresh = refresh;
           b->drop_graphs(););
+}
+
+int acu_batch_set_bp_positions(acu_batch* b, int fbp, int tv) {
+  if (!b) return fail(ACU_ERR_ARG, "null batch");
+  auto ok = [](int v) { return v == ACU_BP_POS_REFERENCE || v == ACU_BP_POS_TILE; };
+  if (!ok(fbp) || !ok(tv)) return fail(ACU_ERR_ARG, "acu_batch_set_bp_positions: unknown positions");
+  // the TV task table points at s.fbp or s.ftv: tvopt rebuilds it
+  ACU_TRY(hipSetDevice(b->ctx->device); b->ctx->sync(); b->bp_refpos = fbp == ACU_BP_POS_REFERENCE;
+          b->tv_bp_refpos = tv == ACU_BP_POS_REFERENCE; b->tv.release(); b->drop_graphs();
+          for (auto& s : b->s) { s.fbp_valid = false; s.tv_valid = false; });
+}
+
+int acu_batch_bp_positions(const acu_batch* b, int* fbp, int* tv) {
+  if (!b) return fail(ACU_ERR_ARG, "null batch");
+  if (fbp) *fbp = b->bp_refpos ? ACU_BP_POS_REFERENCE : ACU_BP_POS_TILE;
+  if (tv) *tv = b->tv_bp_refpos ? ACU_BP_POS_REFERENCE : ACU_BP_POS_TILE;
+  return ACU_OK;
 }
 
 int acu_batch_tv_schedule(const acu_batch* b, int* refresh) {

@@ -41,6 +41,24 @@ constexpr int FPL_THREADS = 256;
 constexpr int AMAX = ACUMEN_FPL_AMAX;  // max angles per lane (1, 2 or 4)
 constexpr int FPL_SB = 8;  // staging loads in flight per lane
 
+// Byte offset 4*(iy*PITCH + ix) of the box cell of a position (xl, yl) with
+// 0 <= xl, yl < 2^16, ix = (int)xl, iy = (int)yl, in three VALU instructions
+// instead of four: ix by a plain conversion, iy converted straight into the
+// high half of the same register (SDWA dst_sel:WORD_1, the low half kept),
+// then one v_dot2_u32_u16 of the packed pair with (4, 4*PITCH), whose result
+// is the LDS byte offset itself.
+template <int PITCH>
+__device__ __forceinline__ unsigned fpl_cell_offset(float xl, float yl) {
+  static_assert(4 * PITCH < 65536, "u16 weight");
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+  unsigned pk = (unsigned)(int)xl;
+  asm("v_cvt_i32_f32_sdwa %0, %1 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE src0_sel:DWORD" : "+v"(pk) : "v"(yl));
+  u16x2 w;
+  w.x = 4;
+  w.y = (unsigned short)(4 * PITCH);
+  return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, pk), w, 0u, false);
+}
+
 // One workgroup's work; PITCH is the LDS row pitch of its boxes (odd, so
 // that the taps of 64 neighbouring rays spread over the banks; which odd
 // value spreads them best depends on the angle, see fpl_pitch_classes).
@@ -93,106 +111,107 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
   }
 
   const ACU_GLOBAL float* __restrict__ img = as_global(tk.img);
-  const FplBoxGeom bg = fpl_box_geom(tk, ab * 4 * AL, 4 * AL, rb * 64, min(N - 1, rb * 64 + 63), lane);
-  for (int c0 = XS; c0 <= XE; c0 += FPL_U) {
-    const int c1 = min(XE, c0 + FPL_U - 1);
-    // bounding box of the chunk's samples (fpl_box: wave-uniform, no LDS)
-    const FplBox B = fpl_box(bg, m, c0, c1);
-    if (!B.any) continue;  // no active lane in this workgroup (uniform)
+  const int r1 = min(N - 1, rb * 64 + 63);
+  const FplBoxGeom bg = fpl_box_geom(tk, ab * 4 * AL, 4 * AL, rb * 64, r1, lane);
+  int c0 = 0, c1 = -1;
+
+  // stage the box B of the chunk [c0, c1] into LDS (all four waves)
+  auto stage = [&](const FplBox& B) {
     const int bx0 = B.x0, by0 = B.y0, bw = B.w, bh = B.h;
-    const bool fits = bw > 0 && bh > 0 && bw <= PITCH && bh * PITCH <= FPL_CAP;
-    if (fits) {
-      // row staging: lanes cover 128 columns, the wave pair (wave >> 1) picks
-      // the row parity, FPL_SB loads in flight per lane; 32-bit byte offsets
-      // from the image base and LDS stores at immediate offsets
-      {
-        const int rp = __builtin_amdgcn_readfirstlane(wave >> 1);
-        const int col = lane + ((wave & 1) << 6);
-        const int gx = bx0 + col;
-        const bool cok = col < bw;
-        const bool xin = cok && gx >= 1 && gx <= N;
-        const ACU_GLOBAL char* ib = (const ACU_GLOBAL char*)img;
-        unsigned goff = 4u * (unsigned)((by0 + rp - 1) * N + (gx - 1));  // wraps for rows above the image
-        const unsigned gstep = 8u * (unsigned)N;
-        float* lrow = box + (rp * PITCH + col);
-        for (int ly0 = rp; ly0 < bh; ly0 += 2 * FPL_SB) {
+    // row staging: lanes cover 128 columns, the wave pair (wave >> 1) picks
+    // the row parity, FPL_SB loads in flight per lane; 32-bit byte offsets
+    // from the image base and LDS stores at immediate offsets
+    const int rp = __builtin_amdgcn_readfirstlane(wave >> 1);
+    const int col = lane + ((wave & 1) << 6);
+    const int gx = bx0 + col;
+    const bool cok = col < bw;
+    const ACU_GLOBAL char* ib = (const ACU_GLOBAL char*)img;
+    unsigned goff = 4u * (unsigned)((by0 + rp - 1) * N + (gx - 1));  // wraps for rows above the image
+    const unsigned gstep = 8u * (unsigned)N;
+    float* lrow = box + (rp * PITCH + col);
+    if (bx0 >= 1 && by0 >= 1 && bx0 + bw - 1 <= N && by0 + bh - 1 <= N) {
+      // the box lies in the image: no per-row tests, whole groups of FPL_SB
+      // rows, then one group whose rows past the last repeat the last (the
+      // same value to the same address)
+      if (cok) {
+        const int nrow = (bh - rp + 1) >> 1;  // rows rp, rp + 2, ... < bh
+        int g = 0;
+        for (; g + FPL_SB <= nrow; g += FPL_SB) {
           float v[FPL_SB];
 #pragma unroll
-          for (int k = 0; k < FPL_SB; ++k) {
-            const int gy = by0 + ly0 + 2 * k;
-            v[k] = 0.0f;
-            if (xin && ly0 + 2 * k < bh && gy >= 1 && gy <= N)
-              v[k] = *(const ACU_GLOBAL float*)(ib + (goff + (unsigned)k * gstep));
-          }
+          for (int k = 0; k < FPL_SB; ++k) v[k] = *(const ACU_GLOBAL float*)(ib + (goff + (unsigned)k * gstep));
 #pragma unroll
-          for (int k = 0; k < FPL_SB; ++k)
-            if (cok && ly0 + 2 * k < bh) lrow[2 * k * PITCH] = v[k];
+          for (int k = 0; k < FPL_SB; ++k) lrow[2 * k * PITCH] = v[k];
           goff += (unsigned)FPL_SB * gstep;
           lrow += 2 * FPL_SB * PITCH;
         }
+        if (g < nrow) {
+          const int last = nrow - 1 - g;
+          float v[FPL_SB];
+#pragma unroll
+          for (int k = 0; k < FPL_SB; ++k)
+            v[k] = *(const ACU_GLOBAL float*)(ib + (goff + (unsigned)min(k, last) * gstep));
+#pragma unroll
+          for (int k = 0; k < FPL_SB; ++k) lrow[2 * min(k, last) * PITCH] = v[k];
+        }
       }
-      __syncthreads();
-      const bool interior = bx0 >= 2 && by0 >= 2 && bx0 + bw - 1 <= N && by0 + bh - 1 <= N;
-      if (interior) {
-        // every cell in the box passes the bounds test: fast positions, no
-        // branches; an inactive (ray, angle) reads a fixed in-box pixel and
-        // its sum is never stored
-        float bxl[AL], byl[AL], cj[AL], sj[AL];
+    } else {
+      const bool xin = cok && gx >= 1 && gx <= N;
+      for (int ly0 = rp; ly0 < bh; ly0 += 2 * FPL_SB) {
+        float v[FPL_SB];
+#pragma unroll
+        for (int k = 0; k < FPL_SB; ++k) {
+          const int gy = by0 + ly0 + 2 * k;
+          v[k] = 0.0f;
+          if (xin && ly0 + 2 * k < bh && gy >= 1 && gy <= N)
+            v[k] = *(const ACU_GLOBAL float*)(ib + (goff + (unsigned)k * gstep));
+        }
+#pragma unroll
+        for (int k = 0; k < FPL_SB; ++k)
+          if (cok && ly0 + 2 * k < bh) lrow[2 * k * PITCH] = v[k];
+        goff += (unsigned)FPL_SB * gstep;
+        lrow += 2 * FPL_SB * PITCH;
+      }
+    }
+  };
+  // this wave's samples of the chunk from the staged box B
+  auto compute = [&](const FplBox& B) {
+    const int bx0 = B.x0, by0 = B.y0, bw = B.w, bh = B.h;
+    const bool interior = bx0 >= 2 && by0 >= 2 && bx0 + bw - 1 <= N && by0 + bh - 1 <= N;
+    if (interior) {
+      // every cell in the box passes the bounds test: fast positions, no
+      // branches; an inactive (ray, angle) reads a fixed in-box pixel and
+      // its sum is never stored
+      float bxl[AL], byl[AL], cj[AL], sj[AL];
+#pragma unroll
+      for (int j = 0; j < AL; ++j) {
+        cj[j] = act[j] ? c[j] : 0.0f;
+        sj[j] = act[j] ? s[j] : 0.0f;
+        bxl[j] = act[j] ? (m - sy[j]) - (float)bx0 : 1.0f;
+        byl[j] = act[j] ? (m + cyv[j]) - (float)by0 : 1.0f;
+      }
+      // u = x - m is an integer below 2^24 in magnitude, so stepping it by
+      // 1.0f gives exactly (float)x - m without a conversion per step
+      float u = (float)c0 - m;
+#pragma unroll 2
+      for (int x = c0; x <= c1; ++x, u += 1.0f) {
+        float val[AL];
 #pragma unroll
         for (int j = 0; j < AL; ++j) {
-          cj[j] = act[j] ? c[j] : 0.0f;
-          sj[j] = act[j] ? s[j] : 0.0f;
-          bxl[j] = act[j] ? (m - sy[j]) - (float)bx0 : 1.0f;
-          byl[j] = act[j] ? (m + cyv[j]) - (float)by0 : 1.0f;
+          const float xl = fmaf(cj[j], u, bxl[j]);
+          const float yl = fmaf(sj[j], u, byl[j]);
+          const float fx = __builtin_amdgcn_fractf(xl), fy = __builtin_amdgcn_fractf(yl);
+          const float* p0 = (const float*)((const char*)box + fpl_cell_offset<PITCH>(xl, yl));
+          const float v00 = p0[0], v10 = p0[1], v01 = p0[PITCH], v11 = p0[PITCH + 1];
+          const float top = fmaf(fx, v10 - v00, v00);
+          const float bot = fmaf(fx, v11 - v01, v01);
+          val[j] = fmaf(fy, bot - top, top);
         }
-        // u = x - m is an integer below 2^24 in magnitude, so stepping it by
-        // 1.0f gives exactly (float)x - m without a conversion per step
-        float u = (float)c0 - m;
-#pragma unroll 2
-        for (int x = c0; x <= c1; ++x, u += 1.0f) {
-          float val[AL];
 #pragma unroll
-          for (int j = 0; j < AL; ++j) {
-            const float xl = fmaf(cj[j], u, bxl[j]);
-            const float yl = fmaf(sj[j], u, byl[j]);
-            const int ix = (int)xl, iy = (int)yl;
-            const float fx = __builtin_amdgcn_fractf(xl), fy = __builtin_amdgcn_fractf(yl);
-            const float* p0 = box + (__mul24(iy, PITCH) + ix);
-            const float v00 = p0[0], v10 = p0[1], v01 = p0[PITCH], v11 = p0[PITCH + 1];
-            const float top = fmaf(fx, v10 - v00, v00);
-            const float bot = fmaf(fx, v11 - v01, v01);
-            val[j] = fmaf(fy, bot - top, top);
-          }
-#pragma unroll
-          for (int j = 0; j < AL; ++j) acc[j] += val[j];
-        }
-      } else {
-        // border chunk: the reference's positions and bounds test
-        for (int x = c0; x <= c1; ++x) {
-          const float u = (float)x - m;
-#pragma unroll
-          for (int j = 0; j < AL; ++j) {
-            if (!act[j]) continue;
-            const float xr = (c[j] * u - sy[j]) + m;
-            const float yr = (s[j] * u + cyv[j]) + m;
-            const float fxr = floorf(xr), fyr = floorf(yr);
-            const int x1 = (int)fxr, y1 = (int)fyr;
-            if (x1 >= 2 && x1 <= N - 1 && y1 >= 2 && y1 <= N - 1) {
-              const float fx = xr - fxr, fy = yr - fyr;
-              const float* p0 = box + (__mul24(y1 - by0, PITCH) + (x1 - bx0));
-              const float v00 = p0[0], v10 = p0[1], v01 = p0[PITCH], v11 = p0[PITCH + 1];
-              const float top = fmaf(fx, v10 - v00, v00);
-              const float bot = fmaf(fx, v11 - v01, v01);
-              acc[j] += fmaf(fy, bot - top, top);
-            }
-          }
-        }
+        for (int j = 0; j < AL; ++j) acc[j] += val[j];
       }
-      __syncthreads();
     } else {
-      // box too large for LDS: bilinear cells from global memory
-      const int P = N + 1;
-      const float fN = (float)N;
+      // border chunk: the reference's positions and bounds test
       for (int x = c0; x <= c1; ++x) {
         const float u = (float)x - m;
 #pragma unroll
@@ -200,15 +219,55 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
           if (!act[j]) continue;
           const float xr = (c[j] * u - sy[j]) + m;
           const float yr = (s[j] * u + cyv[j]) + m;
-          const int cx = (int)__builtin_amdgcn_fmed3f(xr, 0.0f, fN);
-          const int cyi = (int)__builtin_amdgcn_fmed3f(yr, 0.0f, fN);
-          const float wx = __builtin_amdgcn_fractf(xr), wy = __builtin_amdgcn_fractf(yr);
-          const float4 q = global_f4(as_global(tk.cells), (unsigned)(__mul24(cyi, P) + cx));
-          const float t1 = fmaf(wx, q.w, q.z);
-          const float t2 = fmaf(wx, q.y, q.x);
-          acc[j] += fmaf(wy, t1, t2);
+          const float fxr = floorf(xr), fyr = floorf(yr);
+          const int x1 = (int)fxr, y1 = (int)fyr;
+          if (x1 >= 2 && x1 <= N - 1 && y1 >= 2 && y1 <= N - 1) {
+            const float fx = xr - fxr, fy = yr - fyr;
+            const float* p0 = box + (__mul24(y1 - by0, PITCH) + (x1 - bx0));
+            const float v00 = p0[0], v10 = p0[1], v01 = p0[PITCH], v11 = p0[PITCH + 1];
+            const float top = fmaf(fx, v10 - v00, v00);
+            const float bot = fmaf(fx, v11 - v01, v01);
+            acc[j] += fmaf(fy, bot - top, top);
+          }
         }
       }
+    }
+  };
+  // this wave's samples of the chunk, bilinear cells from global memory
+  auto gather = [&]() {
+    const int P = N + 1;
+    const float fN = (float)N;
+    for (int x = c0; x <= c1; ++x) {
+      const float u = (float)x - m;
+#pragma unroll
+      for (int j = 0; j < AL; ++j) {
+        if (!act[j]) continue;
+        const float xr = (c[j] * u - sy[j]) + m;
+        const float yr = (s[j] * u + cyv[j]) + m;
+        const int cx = (int)__builtin_amdgcn_fmed3f(xr, 0.0f, fN);
+        const int cyi = (int)__builtin_amdgcn_fmed3f(yr, 0.0f, fN);
+        const float wx = __builtin_amdgcn_fractf(xr), wy = __builtin_amdgcn_fractf(yr);
+        const float4 q = global_f4(as_global(tk.cells), (unsigned)(__mul24(cyi, P) + cx));
+        const float t1 = fmaf(wx, q.w, q.z);
+        const float t2 = fmaf(wx, q.y, q.x);
+        acc[j] += fmaf(wy, t1, t2);
+      }
+    }
+  };
+  auto fits = [](const FplBox& B) { return B.w > 0 && B.h > 0 && B.w <= PITCH && B.h * PITCH <= FPL_CAP; };
+
+  for (c0 = XS; c0 <= XE; c0 += FPL_U) {
+    c1 = min(XE, c0 + FPL_U - 1);
+    // bounding box of the chunk's samples (fpl_box: wave-uniform, no LDS)
+    const FplBox B = fpl_box(bg, m, c0, c1);
+    if (!B.any) continue;  // no active lane in this workgroup (uniform)
+    if (fits(B)) {
+      stage(B);
+      __syncthreads();
+      compute(B);
+      __syncthreads();
+    } else {
+      gather();  // box too large for LDS
     }
   }
 #pragma unroll

@@ -222,6 +222,18 @@ class Batch:
     def set_bp_tile(self, rows: int) -> None:
         check(self._lib.acu_batch_set_bp_tile(self.handle, int(rows)))
 
+    def set_bp_positions(self, fbp: int, tv: int) -> None:
+        """Backprojection positions (_lib.ACU_BP_POS_REFERENCE or
+        ACU_BP_POS_TILE) of FBP images and inside tvopt's iteration; both
+        default to REFERENCE."""
+        check(self._lib.acu_batch_set_bp_positions(self.handle, int(fbp), int(tv)))
+
+    def bp_positions(self) -> tuple[int, int]:
+        """(FBP positions, tvopt positions)."""
+        f, t = ctypes.c_int(), ctypes.c_int()
+        check(self._lib.acu_batch_bp_positions(self.handle, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
+
     # ---- differentiation (no counterpart in the reference; built on radon) ----
     def radon_adjoint(self, sino) -> np.ndarray:
         """Exact transpose of :meth:`radon` (slice 0): (T,N) sinogram to (N,N) image."""

@@ -219,11 +219,34 @@ int acu_batch_set_fp_variant(acu_batch *b, int slice, int variant);
 /* Backprojection tile of the batch (tv_hpc.F90:107-123 is its arithmetic):
  * 64-pixel-wide tiles of 16, 32 (default) or 64 rows, one workgroup each.
  * The angle blocks (32 angles, each block summed separately) are the same
- * at every height; positions are rounded relative to the tile origin, so
- * heights differ by FP32 rounding only.  The initial value comes from
- * ACUMEN_BP_TILE.  The getter returns the rows or -ACU_ERR_ARG. */
+ * at every height; with the reference's positions the heights give the
+ * same FP32 result, with tile-relative positions they differ by FP32
+ * rounding only.  The initial value comes from ACUMEN_BP_TILE.  The getter
+ * returns the rows or -ACU_ERR_ARG. */
 int acu_batch_set_bp_tile(acu_batch *b, int rows);
 int acu_batch_bp_tile(const acu_batch *b);
+
+/* Backprojection positions (tv_hpc.F90:114).  ACU_BP_POS_REFERENCE forms
+ * yr = s*(x-m) + c*(y-m) + m as the reference does, each product and sum
+ * rounded once in FP32, so every floor, weight and mask decision is the
+ * reference's; ACU_BP_POS_TILE forms it relative to the tile origin with
+ * one FMA per update (rounded at magnitude < 140 instead of N: closer to
+ * FP64, about 25% faster).  fbp: the positions of FBP images, acu_iradon*
+ * and acu_backproject_dev (from ACUMEN_BP_POSITIONS = reference | tile);
+ * tv: those of the backprojections inside tvopt's iteration, whose warm
+ * start is the FBP image (from ACUMEN_TV_BP_POSITIONS).  Both default to
+ * REFERENCE: past ~25 outer iterations tvopt amplifies rounding
+ * differences, and the reference's positions keep its result on the
+ * reference's own FP32 trajectory (3e-5 from it after 50 outer at
+ * 513^2 x 360, against 2e-4 with tile positions inside tvopt, both ~1e-4
+ * from FP64); TILE inside tvopt trades that for ~6% of its throughput.
+ * Setting them invalidates the batch's FBP and TV images.  No counterpart
+ * in the reference. */
+#define ACU_BP_POS_REFERENCE 0
+#define ACU_BP_POS_TILE 1
+int acu_batch_set_bp_positions(acu_batch *b, int fbp, int tv);
+/* *fbp and *tv (either may be NULL) receive the positions; ACU_OK. */
+int acu_batch_bp_positions(const acu_batch *b, int *fbp, int *tv);
 
 /* ---- Differentiation.  No counterpart in the reference: these build on
  * its projector (tv_hpc.F90:129-159) and the periodic differences of its

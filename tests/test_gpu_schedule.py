@@ -141,6 +141,48 @@ def test_50_outer_within_the_references_own_spread(ctx, case513):
     assert e64 < max(1e-4, 1.5 * spread)
 
 
+def test_tv_backprojection_positions(ctx, case513):
+    """Backprojection positions of FBP images and inside tvopt's iteration
+    (acu_batch_set_bp_positions; both the reference's by default).
+    513^2 x 360, 20 outer x 5 CG, all four combinations: the FBP image
+    depends on the FBP positions only (bit for bit); every TV is within the
+    1e-4 bar of the reference's FP32 arithmetic and of FP64; a fresh batch
+    set to the reference's positions everywhere reproduces that TV bit for
+    bit (switching leaves no state behind)."""
+    N, T, th, wpu, f = case513
+    REF, TILE = _lib.ACU_BP_POS_REFERENCE, _lib.ACU_BP_POS_TILE
+    tv32, _, _ = ref.tvopt(f, th, wpu, 0.05, 0.1, 20, 1, 5, "f32")
+    tv64, _, _ = ref.tvopt(f, th, wpu, 0.05, 0.1, 20, 1, 5, "f64")
+    out = {}
+    with Batch(ctx, [(N, T, wpu, th)]) as b:
+        if os.environ.get("ACUMEN_BP_POSITIONS") is None and os.environ.get("ACUMEN_TV_BP_POSITIONS") is None:
+            assert b.bp_positions() == (REF, REF)
+        with pytest.raises(AcumenError):
+            b.set_bp_positions(REF, 2)
+        b.set_sino(0, f)
+        for fp in (REF, TILE):
+            for tp in (REF, TILE):
+                say("bp positions fbp", fp, "tv", tp, N, T, "20 outer")
+                b.set_bp_positions(fp, tp)
+                assert b.bp_positions() == (fp, tp)
+                with pytest.raises(AcumenError):
+                    b.get_image(0, ACU_IMG_TV)  # invalidated by the setting
+                b.tvopt(0.05, 0.1, 20, 1, 5, ACU_TV_GRAPH)
+                out[(fp, tp)] = (b.get_image(0, ACU_IMG_TV), b.get_image(0, ACU_IMG_FBP))
+                e32, e64 = rel(out[(fp, tp)][0], tv32), rel(out[(fp, tp)][0], tv64)
+                print(f"positions fbp {fp} tv {tp}: TV vs reference FP32 {e32:.2e}, vs FP64 {e64:.2e}", flush=True)
+                assert e32 < 1e-4 and e64 < 1e-4
+    for tp in (REF, TILE):
+        assert np.array_equal(out[(REF, tp)][1], out[(REF, REF)][1])
+        assert np.array_equal(out[(TILE, tp)][1], out[(TILE, TILE)][1])
+    assert not np.array_equal(out[(REF, REF)][1], out[(TILE, TILE)][1])
+    with Batch(ctx, [(N, T, wpu, th)]) as b:
+        b.set_sino(0, f)
+        b.set_bp_positions(REF, REF)
+        b.tvopt(0.05, 0.1, 20, 1, 5, ACU_TV_GRAPH)
+        assert np.array_equal(b.get_image(0, ACU_IMG_TV), out[(REF, REF)][0])
+
+
 def test_linear_refresh_and_graph(ctx):
     """refresh = 3 recomputes the carried terms every 3 outer iterations; the
     graph path (two captured bodies) equals the batched path bit for bit, and
