@@ -86,7 +86,10 @@ static_assert(FPL_PITCH + 1 <= 255, "ds_read2 offsets");
 // Row pitches an angle block may use for its boxes (the capacity FPL_CAP is
 // fixed; rows = FPL_CAP / pitch); class FPL_PCLASS_DEFAULT is FPL_PITCH.
 constexpr int FPL_NPITCH = 8;
-constexpr int FPL_PITCHES[FPL_NPITCH] = {93, 97, 99, 101, 103, 105, 107, 109};
+#ifndef ACUMEN_FPL_PITCH_CLASSES
+#define ACUMEN_FPL_PITCH_CLASSES 93, 97, 99, 101, 103, 105, 107, 109
+#endif
+constexpr int FPL_PITCHES[FPL_NPITCH] = {ACUMEN_FPL_PITCH_CLASSES};
 constexpr int FPL_PCLASS_DEFAULT = 3;
 // Cell-layout LDS projector (fplc.hip): the box is FPLC_PITCH x FPLC_ROWS
 // 16-byte bilinear cells (161,600 bytes of the CU's 160 KiB LDS).
