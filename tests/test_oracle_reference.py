@@ -101,3 +101,28 @@ def test_tvopt_equals_reference_live(N, T, lam, nouter, ninner, ncg):
     tvo, fbpo, _ = ref.tvopt(f, th, wpu, 0.05, lam, nouter, ninner, ncg, "f32")
     assert np.array_equal(bits(fbp), bits(fbpo))
     assert np.array_equal(bits(tv), bits(tvo))
+
+
+def test_sign_projections_estimate_rel_l2():
+    """The 64-projection estimate of rel-L2 used for the full-size fixture
+    (tests/golden/headline_2049x1500.npz) is within 35% of the exact value
+    on images of the headline's size, for distances from 1e-6 to 1e-2."""
+    from helpers import rel, rel_projected, sign_projections
+    rng = np.random.default_rng(5)
+    b = synth.phantom(2049, 40, 1001).astype(np.float64)
+    pb = sign_projections(b)
+    for scale in (1e-6, 1e-4, 1e-2):
+        d = rng.standard_normal(b.shape) * scale * np.linalg.norm(b) / b.shape[0]
+        est = rel_projected(sign_projections(b + d), pb)
+        assert abs(est / rel(b + d, b) - 1.0) < 0.35, (scale, est, rel(b + d, b))
+
+
+def test_headline_fixture_records_the_references_own_spread():
+    """The fixture holds the reference's FP32 and FP64 TV of the headline
+    slice: their distance grows from ~1.5e-4 (20 outer) to ~1e-3 (50 outer),
+    which sets the bars of tests/test_gpu_headline.py."""
+    from helpers import rel_projected
+    g = np.load(os.path.join(GOLD, "headline_2049x1500.npz"))
+    s20 = rel_projected(g["tv_f32_20"], g["tv_f64_20"])
+    s50 = rel_projected(g["tv_f32_50"], g["tv_f64_50"])
+    assert 1e-4 < s20 < 3e-4 and 5e-4 < s50 < 2e-3
