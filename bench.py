@@ -191,6 +191,50 @@ def cpu_baseline_tv(ctx, sino, th, wpu, threads):
             "cpu_fp32_vs_fp64_scalars_tv_1_outer_rel_l2": float(f"{rel(tv1, tv1d):.3e}")}
 
 
+def headline_accuracy(ctx, batch, th, wpu, threads):
+    """Checker of the headline's accuracy (part of the CPU baseline leg): the
+    timed batch's configuration (schedule, autotuned kernels) runs tvopt to 20
+    and 50 outer iterations on the slice whose reference results
+    tests/golden/headline_2049x1500.npz keeps -- the reference's own FP32
+    arithmetic (the oracle's FP32 build) and FP64, as 64 Rademacher
+    projections per image.  The slice's sinogram is the oracle's FP32 radon
+    of the 40-ellipse phantom (seed 1001) with Poisson noise (I0 = 5e3, seed
+    1101), regenerated here and matched to the fixture; rel-L2 distances
+    are estimated from the projections (~13% spread)."""
+    import numpy as np
+
+    from acumen_amd import ACU_IMG_TV, ACU_TV_GRAPH, Batch, synth
+    from oracle import ref
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import rel_projected, sign_projections
+
+    g = np.load(os.path.join(ROOT, "tests", "golden", "headline_2049x1500.npz"))
+    N, T = int(g["N"]), int(g["T"])
+    ref.set_threads(threads)
+    try:
+        sino = synth.poisson_noise(ref.radon(synth.phantom(N, 40, 1001), th, "f32"), 5e3, N, 1101)
+    finally:
+        ref.set_threads(1)
+    if not np.array_equal(sign_projections(sino), g["sino"]):
+        return {"error": "the regenerated input differs from the fixture's"}
+    out = {}
+    sched, refresh = batch.tv_schedule()
+    with Batch(ctx, [(N, T, wpu, th)]) as one:
+        one.set_ramp_variant(batch.ramp_variant())
+        one.set_fp_variant(0, batch.fp_variant(0))
+        one.set_bp_tile(batch.bp_tile())
+        one.set_tv_schedule(sched, refresh)
+        one.set_sino(0, sino)
+        for n in (20, 50):
+            one.tvopt(MU, LAM, n, NINNER, NCG, ACU_TV_GRAPH)
+            p = sign_projections(one.get_image(0, ACU_IMG_TV))
+            out[f"tv_{n}_outer"] = {
+                "vs_fp64": float(f"{rel_projected(p, g[f'tv_f64_{n}']):.3e}"),
+                "vs_reference_fp32": float(f"{rel_projected(p, g[f'tv_f32_{n}']):.3e}"),
+                "reference_fp32_vs_fp64": float(f"{rel_projected(g[f'tv_f32_{n}'], g[f'tv_f64_{n}']):.3e}")}
+    return out
+
+
 def run_mixed(args) -> int:
     """64 slices, widths from {512, 768, 1024, 1536, 2048} and U{360..1500}
     angles (seeded), 20-40 ellipses, I0 ~ U[2e3, 2e4], TV as the headline:
@@ -875,6 +919,10 @@ def run_production(args) -> int:
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         threads = cpu_threads(args.cpu_threads)
         cpu = leg("cpu", 75, lambda: cpu_baseline_tv(ctx, sinos[0], th, wpu, threads))
+        if cpu is not None:
+            acc = leg("accuracy", 20, lambda: headline_accuracy(ctx, batch, th, wpu, threads))
+            if acc is not None:
+                cpu["headline_slice_vs_reference_runs"] = acc
 
     batch.close()
     ctx.close()
