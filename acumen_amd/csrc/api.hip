@@ -948,12 +948,16 @@ uint64_t theta_hash(const std::vector<float>& th) {
   return h;
 }
 
-// Average ms per slice of one radon of slice sl (FBP image -> ruk, contents
-// irrelevant to the timing) with variant v, as a batched launch would run
-// it: the cells pass (when v reads it) once, plus the projector launched
-// over R copies of the slice divided by R, R chosen so the launch has at
-// least two workgroups per slot of the GPU (a single small slice would
-// leave CUs idle that a batch fills).  The copies write scratch outputs.
+// Average ms per slice of one radon of slice sl with variant v, as a batched
+// launch would run it: the cells pass (when v reads it) once, plus the
+// projector launched over R copies of the slice divided by R, R the number of
+// the batch's slices of this shape, at least enough for two workgroups per
+// slot of the GPU (a single small slice would leave CUs idle that a batch
+// fills), at most 8.  Each copy reads an image of its own, of ones: the LDS
+// projectors skip boxes of zeros (an FBP image is zero outside the
+// reconstruction disk, a TV iterate soon is not), and copies of one image
+// would share L2 as a batch's slices do not.  The copies write scratch
+// outputs.
 double time_variant(acu_batch* b, int sl, int v, int reps) {
   Slice& s = b->s[sl];
   acu_ctx* c = b->ctx;
@@ -963,19 +967,28 @@ double time_variant(acu_batch* b, int sl, int v, int reps) {
   DevArray<CellTask> cl;
   DevArray<FPLTask> fl;
   DevArray<FPTask> fp;
-  CellTask ct = cell_task(s, s.fbp);
-  ct.block_begin = 0;
-  cl.upload({ct}, b->ctx->stream);
+  struct Scratch { float* p = nullptr; ~Scratch() { if (p) hipFree(p); } } sc, ones;
   int nfl = 0, nfp = 0, nfs = 0, R = 1;
-  struct Scratch { float* p = nullptr; ~Scratch() { if (p) hipFree(p); } } sc;
   if (v > 0) {
     const int per = fpl_blocks(s.N, s.T, v);
     const int slots = v == FPV_CELL ? 256 : 1024;  // workgroups resident on the GPU at once
-    R = std::max(1, std::min(8, (2 * slots + per - 1) / per));
+    int same = 0;
+    for (const auto& o : b->s) same += o.N == s.N && o.T == s.T;
+    R = std::max(1, std::min(8, std::max(same, (2 * slots + per - 1) / per)));
+  }
+  const size_t NN = (size_t)s.N * s.N;
+  ACU_HIP_CHECK(hipMalloc(&ones.p, (size_t)R * NN * sizeof(float)));
+  ACU_HIP_CHECK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(ones.p), 0x3f800000u, (size_t)R * NN,
+                                  c->stream));
+  CellTask ct = cell_task(s, ones.p);
+  ct.block_begin = 0;
+  cl.upload({ct}, b->ctx->stream);
+  if (v > 0) {
+    const int per = fpl_blocks(s.N, s.T, v);
     if (R > 1) ACU_HIP_CHECK(hipMalloc(&sc.p, (size_t)(R - 1) * s.N * s.T * sizeof(float)));
     std::vector<FPLTask> tv;
     for (int r = 0; r < R; ++r) {
-      FPLTask t = fpl_task(s, s.fbp, r == 0 ? s.ruk : sc.p + (size_t)(r - 1) * s.N * s.T);
+      FPLTask t = fpl_task(s, ones.p + (size_t)r * NN, r == 0 ? s.ruk : sc.p + (size_t)(r - 1) * s.N * s.T);
       t.block_begin = r * per;
       tv.push_back(t);
     }

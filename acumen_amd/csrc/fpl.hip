@@ -100,6 +100,7 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
   lu0 = fpl_wmin(lu0);
   lu1 = fpl_wmax(lu1);
   if (lane == 0) { red[wave][0] = lu0; red[wave][1] = lu1; }
+  if (threadIdx.x == 0) { red[0][2] = 0.0f; red[0][3] = 0.0f; }  // nonzero-box flags of the chunks
   __syncthreads();
   const float U0 = fminf(fminf(red[0][0], red[1][0]), fminf(red[2][0], red[3][0]));
   const float U1 = fmaxf(fmaxf(red[0][1], red[1][1]), fmaxf(red[2][1], red[3][1]));
@@ -115,8 +116,10 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
   const FplBoxGeom bg = fpl_box_geom(tk, ab * 4 * AL, 4 * AL, rb * 64, r1, lane);
   int c0 = 0, c1 = -1;
 
-  // stage the box B of the chunk [c0, c1] into LDS (all four waves)
+  // stage the box B of the chunk [c0, c1] into LDS (all four waves);
+  // returns whether this thread staged a value other than +-0
   auto stage = [&](const FplBox& B) {
+    unsigned nzb = 0u;  // OR of the staged values' bits without the sign
     const int bx0 = B.x0, by0 = B.y0, bw = B.w, bh = B.h;
     // row staging: lanes cover 128 columns, the wave pair (wave >> 1) picks
     // the row parity, FPL_SB loads in flight per lane; 32-bit byte offsets
@@ -141,7 +144,10 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
 #pragma unroll
           for (int k = 0; k < FPL_SB; ++k) v[k] = *(const ACU_GLOBAL float*)(ib + (goff + (unsigned)k * gstep));
 #pragma unroll
-          for (int k = 0; k < FPL_SB; ++k) lrow[2 * k * PITCH] = v[k];
+          for (int k = 0; k < FPL_SB; ++k) {
+            lrow[2 * k * PITCH] = v[k];
+            nzb |= __float_as_uint(v[k]) << 1;
+          }
           goff += (unsigned)FPL_SB * gstep;
           lrow += 2 * FPL_SB * PITCH;
         }
@@ -152,7 +158,10 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
           for (int k = 0; k < FPL_SB; ++k)
             v[k] = *(const ACU_GLOBAL float*)(ib + (goff + (unsigned)min(k, last) * gstep));
 #pragma unroll
-          for (int k = 0; k < FPL_SB; ++k) lrow[2 * min(k, last) * PITCH] = v[k];
+          for (int k = 0; k < FPL_SB; ++k) {
+            lrow[2 * min(k, last) * PITCH] = v[k];
+            nzb |= __float_as_uint(v[k]) << 1;
+          }
         }
       }
     } else {
@@ -167,12 +176,15 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
             v[k] = *(const ACU_GLOBAL float*)(ib + (goff + (unsigned)k * gstep));
         }
 #pragma unroll
-        for (int k = 0; k < FPL_SB; ++k)
+        for (int k = 0; k < FPL_SB; ++k) {
           if (cok && ly0 + 2 * k < bh) lrow[2 * k * PITCH] = v[k];
+          nzb |= __float_as_uint(v[k]) << 1;
+        }
         goff += (unsigned)FPL_SB * gstep;
         lrow += 2 * FPL_SB * PITCH;
       }
     }
+    return nzb != 0u;
   };
   // this wave's samples of the chunk from the staged box B
   auto compute = [&](const FplBox& B) {
@@ -256,16 +268,29 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
   };
   auto fits = [](const FplBox& B) { return B.w > 0 && B.h > 0 && B.w <= PITCH && B.h * PITCH <= FPL_CAP; };
 
+  int kst = 0;  // chunks staged so far
   for (c0 = XS; c0 <= XE; c0 += FPL_U) {
     c1 = min(XE, c0 + FPL_U - 1);
     // bounding box of the chunk's samples (fpl_box: wave-uniform, no LDS)
     const FplBox B = fpl_box(bg, m, c0, c1);
     if (!B.any) continue;  // no active lane in this workgroup (uniform)
     if (fits(B)) {
-      stage(B);
+      // A box of zeros (+-0) adds exactly +0 to every sum (with taps of
+      // either sign, fma(fx, v10 - v00, v00) and the lerps after it are +0,
+      // and a sum started at +0 never holds -0), so its samples are
+      // skipped: TV iterates are zero outside the reconstruction disk until
+      // the Laplacian spreads them, a pixel per CG step.  The flag of staged
+      // chunk k is red[0][2 + (k & 1)]; thread 0 clears the other one, which
+      // nobody reads after the previous chunk's last barrier and nobody sets
+      // before this chunk's last barrier.
+      float* flag = &red[0][2 + (kst & 1)];
+      if (stage(B)) *flag = 1.0f;
       __syncthreads();
-      compute(B);
+      const bool nonzero = *flag != 0.0f;
+      if (threadIdx.x == 0) red[0][2 + ((kst + 1) & 1)] = 0.0f;
+      if (nonzero) compute(B);
       __syncthreads();
+      ++kst;
     } else {
       gather();  // box too large for LDS
     }
