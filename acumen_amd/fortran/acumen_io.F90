@@ -129,6 +129,71 @@ contains
     jtheta_e = minloc(abs(f_theta - 180.0), DIM=1)
   end subroutine angle_window
 
+  ! Per-slice done map of a volume run (no counterpart in the reference,
+  ! whose reruns recompute every slice).  Stream file, little-endian, the
+  ! layout acumen_amd/io.py reads and writes: bytes 1-8 "ACUDONE1"; 10 int32
+  ! (nimagef, nimage, ntheta, nsub, nsub_t, wpu, ninner, nouter, ncg, 0);
+  ! 2 float32 (mu, lambda); 8 zero bytes; then one byte per slice, 1 once
+  ! the slice's outputs are written.  With `resume` the slices an earlier
+  ! run with the same header marked stay done; otherwise, or when the file
+  ! is missing, of another length or from other parameters, all start
+  ! pending.  The file is rewritten at once, so a run that stops early can
+  ! be resumed.
+  subroutine done_map_open(path, ints, reals, resume, done)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: ints(10)
+    real, intent(in) :: reals(2)
+    logical, intent(in) :: resume
+    integer(kind=1), intent(out) :: done(:)
+    character(len=8) :: magic
+    integer :: fints(10), rbits(2), fbits(2), u, ios
+    integer(kind=8) :: fsize
+    real :: freals(2)
+    integer(kind=1) :: pad(8), fdone(size(done))
+    logical :: ok
+    done = 0
+    if (resume) then
+      open(newunit=u, file=path, access='stream', form='unformatted', status='old', action='read', iostat=ios)
+      if (ios == 0) then
+        inquire(unit=u, size=fsize)
+        ok = fsize == 64_8 + size(done, kind=8)
+        if (ok) then
+          read(u, pos=1, iostat=ios) magic, fints, freals, pad, fdone
+          rbits = transfer(reals, rbits)
+          fbits = transfer(freals, fbits)
+          ok = ios == 0 .and. magic == "ACUDONE1" .and. all(fints == ints) .and. all(fbits == rbits) &
+               .and. all(pad == 0)
+        end if
+        close(u)
+        if (ok) where (fdone == 1) done = 1
+      end if
+    end if
+    open(newunit=u, file=path, access='stream', form='unformatted', status='replace', action='write', &
+         iostat=ios)
+    if (ios /= 0) then
+      write(IO_WARN_UNIT, *) "acumen: cannot open ", trim(path), " for writing"
+      error stop 3
+    end if
+    pad = 0
+    write(u, pos=1) "ACUDONE1", ints, reals, pad, done
+    close(u)
+  end subroutine done_map_open
+
+  ! Mark slice `job` (1-based) of the done map at `path` as written.
+  subroutine done_map_mark(path, job)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: job
+    integer :: u, ios
+    open(newunit=u, file=path, access='stream', form='unformatted', status='old', action='readwrite', &
+         iostat=ios)
+    if (ios /= 0) then
+      write(IO_WARN_UNIT, *) "acumen: cannot open ", trim(path)
+      error stop 3
+    end if
+    write(u, pos=64_8 + int(job, 8)) 1_1
+    close(u)
+  end subroutine done_map_mark
+
   function wall_seconds() result(t)
     real(kind=8) :: t
     integer(kind=8) :: c, r

@@ -9,18 +9,25 @@
 ! angular sub-sampling, odd padding and every slice -> file/offset map.
 ! Replaced: the MPI master/worker queue (tv_hpc.F90:583-631) by batches of
 ! ACUMEN_BATCH slices (default 16) reconstructed together on the GPU.
+! Added: a per-slice done map, {fbp_tv|fbp_sub_tv}.done beside the outputs,
+! marked as each batch's images are written; ACUMEN_RESUME=1 skips the
+! slices an earlier run with the same parameters marked, and
+! ACUMEN_MAX_BATCHES=k stops after k batches (a run split over jobs).
 program tv_hip
   use Tomography
   use acumen_io
   implicit none
   integer :: nsub, nsub_t, nfiles, n_files_sino, n_files_rec, nimagef, jtheta_s, jtheta_e
-  integer :: jx, jy, js, job, b0, nb, i, batch, ios, jouter
+  integer :: jx, jy, js, job, b0, nb, i, batch, ios, jouter, k, npend, nbatches, max_batches
+  logical :: resume
+  integer, allocatable :: todo(:)
+  integer(kind=1), allocatable :: done(:)
   integer(kind=8) :: offset
   real :: rtmp, rtmp2, rrec
   real(kind=8) :: t0
   real, allocatable :: f_theta(:), sino(:,:), fslice(:,:), img_fbp(:,:), img_tv(:,:), delta(:)
   character(len=150) :: root_dir, exp_name, c_string
-  character(len=:), allocatable :: rdir, info, fbp_name, tv_name
+  character(len=:), allocatable :: rdir, info, fbp_name, tv_name, done_path
 
   call get_environment_variable("ROOT_CT_DIR", root_dir)
   call get_environment_variable("EXP_CT_NAME", exp_name)
@@ -37,6 +44,14 @@ program tv_hip
   if (len_trim(c_string) > 0) then
     read(c_string, *, iostat=ios) batch
     if (ios /= 0 .or. batch < 1) batch = 16
+  end if
+  call get_environment_variable("ACUMEN_RESUME", c_string)
+  resume = trim(c_string) == "1"
+  call get_environment_variable("ACUMEN_MAX_BATCHES", c_string)
+  max_batches = 0
+  if (len_trim(c_string) > 0) then
+    read(c_string, *, iostat=ios) max_batches
+    if (ios /= 0 .or. max_batches < 0) max_batches = 0
   end if
   write(*,*) "Running: ", trim(exp_name)
   write(*,*) " Itr: ", nouter
@@ -79,12 +94,24 @@ program tv_hip
   allocate(img_fbp(nimage, nimage), img_tv(nimage, nimage), delta(max(nouter, 1)))
   rtmp2 = real(nimagef) / real(n_files_sino)
   rrec = real(nimagef) / real(n_files_rec)
-  call batch_init(min(batch, nimagef))
-  do b0 = 1, nimagef, batch
-    nb = min(batch, nimagef - b0 + 1)
-    if (nb /= min(batch, nimagef)) call batch_init(nb)
+  allocate(done(nimagef))
+  done_path = rdir // tv_name // ".done"
+  call done_map_open(done_path, [nimagef, nimage, ntheta, nsub, nsub_t, wpu, ninnner, nouter, ncg, 0], &
+                     [mu, lambda], resume, done)
+  todo = pack([(k, k = 1, nimagef)], done == 0)
+  npend = size(todo)
+  if (npend < nimagef) write(*,*) "Resuming:", nimagef - npend, "slices done earlier"
+  if (npend > 0) call batch_init(min(batch, npend))
+  nbatches = 0
+  do b0 = 1, npend, batch
+    if (max_batches > 0 .and. nbatches >= max_batches) then
+      write(*,*) "Stopped after", nbatches, "batches;", npend - b0 + 1, "slices pending (ACUMEN_RESUME=1 continues)"
+      exit
+    end if
+    nb = min(batch, npend - b0 + 1)
+    if (nb /= min(batch, npend)) call batch_init(nb)
     do i = 1, nb
-      job = b0 + i - 1
+      job = todo(b0 + i - 1)
       js = ceiling(job / rtmp2)
       offset = int(job - floor((js - 1) * rtmp2) - 1, 8) * int(nimagef + add_Nx, 8) * int(nfiles, 8)
       call read_reals_at(rdir // numbered("sinograms", js), offset, (nimagef + add_Nx) * nfiles, sino)
@@ -97,7 +124,7 @@ program tv_hip
     end do
     call batch_tvopt()
     do i = 1, nb
-      job = b0 + i - 1
+      job = todo(b0 + i - 1)
       call batch_get(i, ACU_IMG_FBP, img_fbp)
       call batch_get(i, ACU_IMG_TV, img_tv)
       if (nouter > 0) then
@@ -110,8 +137,10 @@ program tv_hip
       offset = int(job - floor((js - 1) * rrec) - 1, 8) * int(nimage, 8) * int(nimage, 8)
       call write_reals_at(rdir // numbered(fbp_name, js), offset, nimage * nimage, img_fbp)
       call write_reals_at(rdir // numbered(tv_name, js), offset, nimage * nimage, img_tv)
+      call done_map_mark(done_path, job)
     end do
-    write(*,*) "Slices", b0, "to", b0 + nb - 1, "done"
+    nbatches = nbatches + 1
+    write(*,*) "Slices", todo(b0), "to", todo(b0 + nb - 1), "done"
   end do
   call fin_radon()
   write(*,*) "Total Compute time: ", real(wall_seconds() - t0)

@@ -162,3 +162,58 @@ def angle_window(theta_deg):
     and first argmin |theta - 180| (tv_hpc.F90:546-547)."""
     th = _f32(theta_deg)
     return int(np.argmin(np.abs(th))), int(np.argmin(np.abs(th - np.float32(180.0))))
+
+
+# Per-slice done map of a volume run (no counterpart in the reference, whose
+# reruns recompute every slice: tv_hpc.F90:583-631 keeps no record of what is
+# written).  File ``{prefix}.done`` beside the outputs, ``prefix`` the last
+# output kind the run writes (fbp_tv / fbp_sub_tv, or fbp / fbp_sub without
+# TV).  Layout, little-endian, the same for acumen_amd/fortran/acumen_io.F90:
+#   bytes  0-7   b"ACUDONE1"
+#   bytes  8-47  int32 nimagef, nimage, ntheta, nsub, nsub_t, wpu, ninner,
+#                nouter, ncg, 0
+#   bytes 48-55  float32 mu, lambda
+#   bytes 56-63  zero
+#   byte  64+s-1 1 once slice s (1-based) has its outputs written, else 0
+DONE_MAGIC = b"ACUDONE1"
+DONE_HEADER = 64
+
+
+def done_header(nimagef: int, nimage: int, ntheta: int, nsub: int, nsub_t: int, wpu: int,
+                ninner: int, nouter: int, ncg: int, mu: float, lam: float) -> bytes:
+    ints = np.array([nimagef, nimage, ntheta, nsub, nsub_t, wpu, ninner, nouter, ncg, 0], "<i4")
+    reals = np.array([mu, lam], "<f4")
+    h = DONE_MAGIC + ints.tobytes() + reals.tobytes()
+    return h + bytes(DONE_HEADER - len(h))
+
+
+class DoneMap:
+    """The done map of one volume run.  ``resume`` keeps the slices an
+    earlier run with the same header marked done; otherwise, or when the
+    file is missing, short or from a run with other parameters, every slice
+    starts pending.  The file is (re)written at once, so a run that stops
+    early can be resumed."""
+
+    def __init__(self, path: str, header: bytes, nimagef: int, resume: bool):
+        self.path = path
+        done = np.zeros(nimagef, np.uint8)
+        if resume and os.path.exists(path):
+            raw = np.fromfile(path, np.uint8)
+            if raw.size == DONE_HEADER + nimagef and raw[:DONE_HEADER].tobytes() == header:
+                done = (raw[DONE_HEADER:] == 1).astype(np.uint8)
+        self.done = done
+        with open(path, "wb") as fh:
+            fh.write(header)
+            fh.write(done.tobytes())
+
+    def pending(self):
+        """1-based slices not yet done, in order."""
+        return [int(s) + 1 for s in np.flatnonzero(self.done == 0)]
+
+    def mark(self, slices) -> None:
+        """Record slices (1-based) whose outputs are written."""
+        with open(self.path, "r+b") as fh:
+            for s in slices:
+                self.done[s - 1] = 1
+                fh.seek(DONE_HEADER + s - 1)
+                fh.write(b"\x01")

@@ -61,7 +61,8 @@ def _write_at(path: str, offset_elems: int, a: np.ndarray) -> None:
 
 
 def reconstruct_dataset(ctx, root: str, exp: str, *, nsub: int = 1, nsub_t: int = 1, wpu: int = 64,
-                        tv=None, batch: int = 16, tune: bool = True, threads: int = 8, log=None):
+                        tv=None, batch: int = 16, tune: bool = True, threads: int = 8, log=None,
+                        resume: bool = False, chunk: int = 0, max_chunks: int = 0):
     """Every slice of the dataset $ROOT_CT_DIR/$EXP_CT_NAME streamed through
     the GPU with the file contract of tomography/tv_hpc.F90:392-652 (and of
     acumen_amd/fortran/tv_hip.F90): header offsets 0-3 of
@@ -71,10 +72,15 @@ def reconstruct_dataset(ctx, root: str, exp: str, *, nsub: int = 1, nsub_t: int 
     written at the reference's 64-bit element offsets.  Without ``tv`` only
     the FBP files are written.
 
-    Slices are read and written by a pool of ``threads`` host threads and
-    reconstructed by ``reconstruct`` (two lanes overlapping PCIe, staging
-    and kernels).  Returns the ``reconstruct`` statistics plus the read and
-    write seconds."""
+    Slices go in chunks of ``chunk`` (0: all at once): a pool of ``threads``
+    host threads reads a chunk's sinograms, ``reconstruct`` runs it (two
+    lanes overlapping PCIe, staging and kernels), the pool writes its images
+    and the chunk's slices are marked in the done map ``{fbp_tv|fbp}.done``
+    (``io.DoneMap``).  ``resume`` skips the slices an earlier run with the
+    same parameters marked done; ``max_chunks`` (0: no limit) stops after
+    that many chunks, for runs split over several jobs.  Returns the
+    summed ``reconstruct`` statistics of this call plus the read and write
+    seconds, the slices reconstructed and the slices still pending."""
     import math
     import time
     from concurrent.futures import ThreadPoolExecutor
@@ -94,42 +100,67 @@ def reconstruct_dataset(ctx, root: str, exp: str, *, nsub: int = 1, nsub_t: int 
         raise ValueError("the angle window holds no projection")
     th = theta[js0 + np.arange(ntheta) * nsub_t]
     rowlen = width * nfiles
-    sinos = np.empty((nimagef, ntheta, nimage), np.float32)
+    sub = nsub + nsub_t != 2
+    names = ("fbp_sub", "fbp_sub_tv") if sub else ("fbp", "fbp_tv")
+    nn = nimage * nimage
+    if tv is None:
+        dh = aio.done_header(nimagef, nimage, ntheta, nsub, nsub_t, wpu, 0, 0, 0, 0.0, 0.0)
+    else:
+        q = dict(ninner=1, ncg=5)
+        q.update(tv)
+        dh = aio.done_header(nimagef, nimage, ntheta, nsub, nsub_t, wpu, q["ninner"], q["nouter"], q["ncg"],
+                             q["mu"], q["lam"])
+    done = aio.DoneMap(ds.path(names[1 if tv is not None else 0] + ".done"), dh, nimagef, resume)
+    todo = done.pending()
+    step = max(1, len(todo) if chunk <= 0 else chunk)
 
-    def read(job):
+    def read(into, i, job):
         js, off = aio.slice_location(job, nimagef, n_files_sino, rowlen)
         a = np.fromfile(ds.path(f"sinograms{js}.dat"), np.float32, rowlen, offset=4 * off)
         if a.size != rowlen:
             raise IOError(f"sinograms{js}.dat: short read for slice {job}")
         a = a.reshape(nfiles, width)
-        sinos[job - 1] = a[js0::nsub_t][:ntheta, ::nsub][:, :nimage]
+        into[i] = a[js0::nsub_t][:ntheta, ::nsub][:, :nimage]
 
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(max(1, threads)) as ex:
-        list(ex.map(read, range(1, nimagef + 1)))
-    t_read = time.perf_counter() - t0
-    if log:
-        log(f"{nimagef} slices of {nimage}^2 x {ntheta} angles read in {t_read:.2f} s")
-    fbp, tvi, stats = reconstruct(ctx, sinos, th, wpu, tv=tv, fbp=True, batch=batch, tune=tune)
-    sub = nsub + nsub_t != 2
-    names = ("fbp_sub", "fbp_sub_tv") if sub else ("fbp", "fbp_tv")
-    nn = nimage * nimage
-
-    def write(job):
+    def write(fbp, tvi, i, job):
         js, off = aio.slice_location(job, nimagef, n_files_rec, nn)
-        _write_at(ds.path(f"{names[0]}{js}.dat"), off, fbp[job - 1])
+        _write_at(ds.path(f"{names[0]}{js}.dat"), off, fbp[i])
         if tvi is not None:
-            _write_at(ds.path(f"{names[1]}{js}.dat"), off, tvi[job - 1])
+            _write_at(ds.path(f"{names[1]}{js}.dat"), off, tvi[i])
 
-    t0 = time.perf_counter()
-    # slices of one file are written by one thread (no two writers on a file)
-    by_file = {}
-    for job in range(1, nimagef + 1):
-        by_file.setdefault(aio.slice_location(job, nimagef, n_files_rec, nn)[0], []).append(job)
+    stats = dict(ms=0.0, setup_ms=0.0)
+    t_read = t_write = 0.0
+    nrec = 0
     with ThreadPoolExecutor(max(1, threads)) as ex:
-        list(ex.map(lambda jobs: [write(j) for j in jobs], by_file.values()))
-    t_write = time.perf_counter() - t0
-    stats.update(read_s=t_read, write_s=t_write, nimage=nimage, ntheta=ntheta, slices=nimagef)
+        for k, c0 in enumerate(range(0, len(todo), step)):
+            if max_chunks and k >= max_chunks:
+                break
+            jobs = todo[c0:c0 + step]
+            sinos = np.empty((len(jobs), ntheta, nimage), np.float32)
+            t0 = time.perf_counter()
+            list(ex.map(lambda a: read(sinos, *a), enumerate(jobs)))
+            t_read += time.perf_counter() - t0
+            if log:
+                log(f"{len(jobs)} slices of {nimage}^2 x {ntheta} angles read ({nrec + len(jobs)} of "
+                    f"{len(todo)} pending)")
+            fbp, tvi, st = reconstruct(ctx, sinos, th, wpu, tv=tv, fbp=True, batch=batch, tune=tune)
+            stats["ms"] += st["ms"]
+            stats["setup_ms"] += st["setup_ms"]
+            t0 = time.perf_counter()
+            # slices of one file are written by one thread (no two writers on a file)
+            by_file = {}
+            for i, job in enumerate(jobs):
+                by_file.setdefault(aio.slice_location(job, nimagef, n_files_rec, nn)[0], []).append((i, job))
+            list(ex.map(lambda items: [write(fbp, tvi, i, job) for i, job in items], by_file.values()))
+            t_write += time.perf_counter() - t0
+            done.mark(jobs)
+            nrec += len(jobs)
+    sec = stats["ms"] / 1e3
+    stream = sec - stats["setup_ms"] / 1e3
+    stats.update(slices_per_s=nrec / sec if sec > 0 else 0.0,
+                 stream_slices_per_s=nrec / stream if stream > 0 else 0.0,
+                 read_s=t_read, write_s=t_write, nimage=nimage, ntheta=ntheta, slices=nrec,
+                 pending=len(todo) - nrec, skipped=nimagef - len(todo))
     return stats
 
 
@@ -147,6 +178,10 @@ def main(argv=None) -> int:
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="slices per chunk; each chunk's slices are marked in the done map once written")
+    ap.add_argument("--resume", action="store_true", help="skip slices an earlier run marked done")
+    ap.add_argument("--max-chunks", type=int, default=0, help="stop after this many chunks (0: no limit)")
     a = ap.parse_args(argv)
     env = os.environ
     nouter = int(env.get("Nouter_Loops", "0"))
@@ -159,9 +194,11 @@ def main(argv=None) -> int:
                                  nsub=int(env.get("Sub_Sample_Radial", "1")),
                                  nsub_t=int(env.get("Sub_Sample_Theta", "1")),
                                  wpu=int(env.get("Radon_wpu", "64")), tv=tv, batch=a.batch,
-                                 tune=not a.no_tune, threads=a.threads, log=print)
+                                 tune=not a.no_tune, threads=a.threads, log=print, resume=a.resume,
+                                 chunk=a.chunk, max_chunks=a.max_chunks)
     print(f"{st['slices']} slices of {st['nimage']}^2 x {st['ntheta']} angles: read {st['read_s']:.2f} s, "
-          f"reconstruction {st['ms'] / 1e3:.2f} s ({st['slices_per_s']:.1f} slices/s), write {st['write_s']:.2f} s")
+          f"reconstruction {st['ms'] / 1e3:.2f} s ({st['slices_per_s']:.1f} slices/s), write {st['write_s']:.2f} s; "
+          f"{st['skipped']} done earlier, {st['pending']} pending")
     return 0
 
 

@@ -139,6 +139,84 @@ def test_streamed_dataset_equals_batch_path(ctx, tmp_path, nsub, nsub_t):
             assert np.array_equal(got_fbp, b.get_image(0, ACU_IMG_FBP)), s
 
 
+def _sentinel(ds, prefix, s, nimagef, nn, n_files_rec=1):
+    """Overwrite 1-based slice s of {prefix}{js}.dat with -7."""
+    js, off = aio.slice_location(s, nimagef, n_files_rec, nn)
+    with open(ds.path(f"{prefix}{js}.dat"), "r+b") as fh:
+        fh.seek(4 * off)
+        fh.write(np.full(nn, -7.0, np.float32).tobytes())
+
+
+def test_tv_volume_driver_resume(ctx, volume):
+    """tv_hip with ACUMEN_MAX_BATCHES=1 stops after one batch and marks its
+    slices in fbp_tv.done; ACUMEN_RESUME=1 then reconstructs only the other
+    slices (a sentinel written into a done slice survives), and the volume
+    equals an uninterrupted run's bit for bit."""
+    ds, th, sinos = volume
+    N = NIMAGEF + 1
+    nn = N * N
+    env = tv_env(1, 1)  # batches of 12: 12 + 12 + 8 slices
+    run("tv_hip", ds, env)
+    full_tv = aio.read_images(ds, "fbp_tv", NIMAGEF, N)
+    full_fbp = aio.read_images(ds, "fbp", NIMAGEF, N)
+    assert np.array_equal(np.fromfile(ds.path("fbp_tv.done"), np.uint8)[64:], np.ones(NIMAGEF, np.uint8))
+    for f in ("fbp1.dat", "fbp_tv1.dat"):
+        os.remove(ds.path(f))
+    out = run("tv_hip", ds, dict(env, ACUMEN_MAX_BATCHES="1"))
+    assert "Stopped after" in out
+    done = np.fromfile(ds.path("fbp_tv.done"), np.uint8)[64:]
+    assert list(done) == [1] * 12 + [0] * (NIMAGEF - 12)
+    _sentinel(ds, "fbp_tv", 4, NIMAGEF, nn)
+    out = run("tv_hip", ds, dict(env, ACUMEN_RESUME="1"))
+    assert "Resuming:" in out
+    assert np.array_equal(np.fromfile(ds.path("fbp_tv.done"), np.uint8)[64:], np.ones(NIMAGEF, np.uint8))
+    got_tv = aio.read_images(ds, "fbp_tv", NIMAGEF, N)
+    assert (got_tv[3] == -7.0).all()
+    got_tv[3] = full_tv[3]
+    assert np.array_equal(got_tv, full_tv)
+    assert np.array_equal(aio.read_images(ds, "fbp", NIMAGEF, N), full_fbp)
+    # a run with other parameters does not take the marks: everything is recomputed
+    out = run("tv_hip", ds, dict(env, ACUMEN_RESUME="1", Opt_ncg=str(NCG + 1)))
+    assert "Resuming:" not in out
+
+
+def test_streamed_dataset_resume(ctx, tmp_path):
+    """reconstruct_dataset in chunks of 10 with max_chunks=2 reconstructs 20
+    slices and marks them done; resume=True reconstructs the other 12 only
+    (a sentinel written into a done slice survives), and the files equal an
+    uninterrupted run's bit for bit."""
+    from acumen_amd import volume
+    say("streamed dataset resume")
+    th = np.arange(NFILES, dtype=np.float32) * np.float32(5.0)
+    N = NIMAGEF + 1
+    nn = N * N
+    sinos = np.stack([ref.radon(disk_phantom(N, 4, 700 + s), th, "f64").astype(np.float32)
+                      for s in range(NIMAGEF)])
+    ds = aio.Dataset(str(tmp_path), "resume")
+    aio.write_dataset(ds, th, sinos, n_files_sino=3, n_files_rec=2)
+    tv = dict(mu=MU, lam=LAM, nouter=NOUTER, ninner=1, ncg=NCG)
+    kw = dict(wpu=WPU, tv=tv, batch=5, tune=False, threads=4)
+    st = volume.reconstruct_dataset(ctx, ds.root, ds.name, **kw)
+    assert (st["slices"], st["pending"], st["skipped"]) == (NIMAGEF, 0, 0)
+    full_tv = aio.read_images(ds, "fbp_tv", NIMAGEF, N, n_files_rec=2)
+    full_fbp = aio.read_images(ds, "fbp", NIMAGEF, N, n_files_rec=2)
+    for f in ("fbp1.dat", "fbp2.dat", "fbp_tv1.dat", "fbp_tv2.dat"):
+        os.remove(ds.path(f))
+    st = volume.reconstruct_dataset(ctx, ds.root, ds.name, chunk=10, max_chunks=2, **kw)
+    assert (st["slices"], st["pending"], st["skipped"]) == (20, NIMAGEF - 20, 0)
+    _sentinel(ds, "fbp_tv", 5, NIMAGEF, nn, n_files_rec=2)
+    st = volume.reconstruct_dataset(ctx, ds.root, ds.name, chunk=10, resume=True, **kw)
+    assert (st["slices"], st["pending"], st["skipped"]) == (NIMAGEF - 20, 0, 20)
+    got_tv = aio.read_images(ds, "fbp_tv", NIMAGEF, N, n_files_rec=2)
+    assert (got_tv[4] == -7.0).all()
+    got_tv[4] = full_tv[4]
+    assert np.array_equal(got_tv, full_tv)
+    assert np.array_equal(aio.read_images(ds, "fbp", NIMAGEF, N, n_files_rec=2), full_fbp)
+    # nothing left: a resumed run reconstructs nothing
+    st = volume.reconstruct_dataset(ctx, ds.root, ds.name, resume=True, **kw)
+    assert (st["slices"], st["pending"], st["skipped"]) == (0, 0, NIMAGEF)
+
+
 def test_projections_to_sinogram_driver(ctx, tmp_path):
     # angles without an exact 0 / 180 (both end projections interpolated),
     # two projection and sinogram files, even n (Nw = n + 1)
