@@ -206,9 +206,11 @@ constexpr int BP_ROWTERM_PAD = 64;  // floats after the table (the last tile's r
 void launch_bp_rowterm(const float2* sc, int N, int T, float mid, float* rowterm, hipStream_t s);
 void launch_forward(const FPTask* d_tasks, int ntasks, int base, int blocks, int sum_base,
                     int sum_blocks, hipStream_t s);
-// al = 1, 2, 4: fpl.hip; al = FPV_CELL: fplc.hip
+// al = 1, 2, 4: fpl.hip; al = FPV_CELL: fplc.hip.  order (al = 1, 2, 4;
+// null: block i is block base + i): the launch's i-th logical block is block
+// base + order[i], a permutation of [0, blocks) from fpl_dispatch_order.
 void launch_forward_lds(const FPLTask* d_tasks, int ntasks, int base, int blocks, int al,
-                        hipStream_t s);
+                        const int* order, hipStream_t s);
 void launch_forward_cell(const FPLTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
 
 // Host-side helpers.
@@ -221,6 +223,15 @@ int fp_chunks(int N, int clen);
 int fp_sum_blocks(int N, int T);
 int fpl_angles_per_lane(int N, int T);
 int fpl_blocks(int N, int T, int al);
+// One slice of an LDS projector launch: side N, nab angle blocks, its first
+// block (blocks rb * nab + ab from there, rb = ray block of 64 rays).
+struct FplGeom { int N, nab, block_begin; };
+// Dispatch order of a launch over blocks [base, base + n) of the slices g:
+// a permutation of [0, n) that gives each XCD's share of the grid (the
+// logical blocks [k*C, (k+1)*C), C = xcd_grid(n) / XCDS, that xcd_remap
+// hands XCD k) about the same estimated work, longest workgroups first.  A
+// ray block's angle blocks stay consecutive.
+std::vector<int> fpl_dispatch_order(const std::vector<FplGeom>& g, int base, int n);
 int fplc_blocks(int N, int T);
 // angles per workgroup block of an LDS projector variant
 inline int fpl_angle_block(int al) { return al == FPV_CELL ? 16 : 4 * al; }

@@ -252,6 +252,11 @@ struct acu_batch {
   int rf_total = 0, bp_total = 0, cl_total = 0, fp_total = 0, fs_total = 0, ew_total = 0,
       sw_total = 0;
   int fl_total[acu::FL_GROUPS] = {0, 0, 0, 0};  // LDS projector blocks per variant group (lds1, lds2, lds4, cell16)
+  // dispatch orders of the LDS projector (fpl_dispatch_order) per variant
+  // group: of the launch over all slices, and of each slice's own launch at
+  // [fl_begin, fl_begin + fl_n); empty when ACUMEN_FPL_ORDER=rb
+  acu::DevArray<int> fl_ord_all[acu::FL_GROUPS], fl_ord_one[acu::FL_GROUPS];
+  bool fpl_order = true;
   int ramp_variant = acu::RAMP_FP64;  // arithmetic of the ramp filter
   int bp_ty = acu::BP_TY_DEFAULT;     // backprojection tile height (16, 32 or 64)
   bool bp_refpos = true;              // FBP backprojection positions in the reference's arithmetic
@@ -293,6 +298,8 @@ struct acu_batch {
     rf_f.release(); rf_fk.release(); rf_ruk.release(); rf_rp.release();
     bp_fbp.release(); bp_ftv.release(); bp_ifkt.release(); bp_iukt.release(); bp_irpt.release();
     fp_uk.release(); fp_p.release(); tv.release();
+    for (auto& o : fl_ord_all) o.release();
+    for (auto& o : fl_ord_one) o.release();
     fp_g.release(); res_g.release(); adj_g.release();
     tvg_g.release(); loss_g.release();
     if (grad_mem) hipFree(grad_mem);
@@ -500,6 +507,30 @@ void assign_fp(acu_batch* b) {
   }
   b->cl_total = cl; b->fp_total = fp; b->fs_total = fs;
   for (int g = 0; g < FL_GROUPS; ++g) b->fl_total[g] = fl[g];
+  // dispatch orders (the cell variant keeps its own order)
+  for (int g = 0; g < FL_GROUPS; ++g) {
+    b->fl_ord_all[g].release();
+    b->fl_ord_one[g].release();
+    if (!b->fpl_order || g == fl_group(FPV_CELL) || fl[g] == 0) continue;
+    std::vector<FplGeom> geo;
+    std::vector<int> one((size_t)fl[g]);
+    for (const auto& s : b->s) {
+      if (s.fpv <= 0 || fl_group(s.fpv) != g) continue;
+      const FplGeom t{s.N, (s.T + fpl_angle_block(s.fpv) - 1) / fpl_angle_block(s.fpv), s.fl_begin};
+      geo.push_back(t);
+      const std::vector<int> o = fpl_dispatch_order({t}, s.fl_begin, s.fl_n);
+      std::copy(o.begin(), o.end(), one.begin() + s.fl_begin);
+    }
+    b->fl_ord_all[g].upload(fpl_dispatch_order(geo, 0, fl[g]), b->ctx->stream);
+    b->fl_ord_one[g].upload(one, b->ctx->stream);
+  }
+}
+
+// dispatch order of an LDS projector launch (null: ray blocks in order)
+const int* fl_order_all(const acu_batch* b, int g) { return b->fl_ord_all[g].d; }
+const int* fl_order_one(const acu_batch* b, const Slice& s) {
+  const auto& o = b->fl_ord_one[fl_group(s.fpv)];
+  return o.d ? o.d + s.fl_begin : nullptr;
 }
 
 template <class ImgF, class OutF>
@@ -661,7 +692,8 @@ void radon_seq(acu_batch* b, const FPSet& set, int sl) {
     for (int g = 0; g < FL_GROUPS; ++g)
       if (b->fl_total[g] > 0)
         timed(c, K_FP, [&] {
-          launch_forward_lds(set.fl[g].d, set.fl[g].n, 0, b->fl_total[g], al_of[g], c->stream);
+          launch_forward_lds(set.fl[g].d, set.fl[g].n, 0, b->fl_total[g], al_of[g], fl_order_all(b, g),
+                             c->stream);
         });
     if (b->fp_total > 0)
       timed(c, K_FP, [&] { launch_forward(set.fp.d, set.fp.n, 0, b->fp_total, 0, b->fs_total, c->stream); });
@@ -672,7 +704,7 @@ void radon_seq(acu_batch* b, const FPSet& set, int sl) {
     if (s.fpv > 0) {
       const int g = fl_group(s.fpv);
       timed(c, K_FP, [&] {
-        launch_forward_lds(set.fl[g].d, set.fl[g].n, s.fl_begin, s.fl_n, s.fpv, c->stream);
+        launch_forward_lds(set.fl[g].d, set.fl[g].n, s.fl_begin, s.fl_n, s.fpv, fl_order_one(b, s), c->stream);
       });
     } else {
       timed(c, K_FP, [&] {
@@ -901,7 +933,7 @@ void radon_dev(acu_batch* b, int sl, float* sino, const float* img) {
     FPLTask t = fpl_task(s, img, sino);
     t.block_begin = 0;
     FPLTask* d = put_adhoc(b, t);
-    timed(c, K_FP, [&] { launch_forward_lds(d, 1, 0, s.fl_n, s.fpv, c->stream); });
+    timed(c, K_FP, [&] { launch_forward_lds(d, 1, 0, s.fl_n, s.fpv, fl_order_one(b, s), c->stream); });
   } else {
     FPTask t = fp_task(s, sino);
     t.block_begin = 0;
@@ -967,6 +999,7 @@ double time_variant(acu_batch* b, int sl, int v, int reps) {
   DevArray<CellTask> cl;
   DevArray<FPLTask> fl;
   DevArray<FPTask> fp;
+  DevArray<int> ord;
   struct Scratch { float* p = nullptr; ~Scratch() { if (p) hipFree(p); } } sc, ones;
   int nfl = 0, nfp = 0, nfs = 0, R = 1;
   if (v > 0) {
@@ -994,6 +1027,11 @@ double time_variant(acu_batch* b, int sl, int v, int reps) {
     }
     fl.upload(tv, b->ctx->stream);
     nfl = R * per;
+    if (b->fpl_order && v != FPV_CELL) {
+      std::vector<FplGeom> geo;
+      for (const FPLTask& t : tv) geo.push_back({t.N, t.nab, t.block_begin});
+      ord.upload(fpl_dispatch_order(geo, 0, nfl), b->ctx->stream);
+    }
   } else {
     FPTask t = fp_task(s, s.ruk);
     t.block_begin = 0;
@@ -1018,9 +1056,9 @@ double time_variant(acu_batch* b, int sl, int v, int reps) {
   };
   double ms = 0.0;
   if (cells) ms += timed_ms([&] { launch_cells(cl.d, 1, 0, cell_blocks(s.N), c->stream); });
-  if (v > 0) ms += timed_ms([&] { launch_forward_lds(fl.d, R, 0, nfl, v, c->stream); }) / R;
+  if (v > 0) ms += timed_ms([&] { launch_forward_lds(fl.d, R, 0, nfl, v, ord.d, c->stream); }) / R;
   else ms += timed_ms([&] { launch_forward(fp.d, 1, 0, nfp, 0, nfs, c->stream); });
-  cl.release(); fl.release(); fp.release();
+  cl.release(); fl.release(); fp.release(); ord.release();
   s.fpv = saved;
   return ms;
 }
@@ -1392,6 +1430,9 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
     // tile inside tvopt's iteration
     if (const char* pv2 = std::getenv("ACUMEN_BP_POSITIONS")) b->bp_refpos = std::strcmp(pv2, "tile") != 0;
     if (const char* pv3 = std::getenv("ACUMEN_TV_BP_POSITIONS")) b->tv_bp_refpos = std::strcmp(pv3, "tile") != 0;
+    // LDS projector dispatch: ACUMEN_FPL_ORDER = balanced (default:
+    // fpl_dispatch_order) | rb (ray blocks in order)
+    if (const char* po = std::getenv("ACUMEN_FPL_ORDER")) b->fpl_order = std::strcmp(po, "rb") != 0;
     // backprojection tile height: ACUMEN_BP_TILE = 16 | 32 (default) | 64
     if (const char* tv = std::getenv("ACUMEN_BP_TILE")) {
       const int ty = std::atoi(tv);

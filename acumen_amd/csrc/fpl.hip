@@ -302,12 +302,13 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
 
 template <int AL>
 __global__ __launch_bounds__(FPL_THREADS) void fpl_kernel(const FPLTask* __restrict__ tasks, int ntasks,
-                                                          int base, int nblocks) {
+                                                          int base, int nblocks,
+                                                          const int* __restrict__ order) {
   __shared__ float box[FPL_CAP];
   __shared__ float red[4][4];  // per-wave min x, min y, max x, max y (or step range)
   const int lbk = xcd_remap(blockIdx.x, gridDim.x);
   if (lbk >= nblocks) return;
-  const int gb = base + lbk;
+  const int gb = base + (order ? order[lbk] : lbk);
   const FPLTask& tk = tasks[fpl_find_task(tasks, ntasks, gb)];
   const int ab = (gb - tk.block_begin) % tk.nab;
   switch (tk.pclass ? (int)tk.pclass[ab] : FPL_PCLASS_DEFAULT) {
@@ -440,16 +441,91 @@ int fpl_blocks(int N, int T, int al) {
 }
 
 void launch_forward_lds(const FPLTask* d_tasks, int ntasks, int base, int blocks, int al,
-                        hipStream_t s) {
+                        const int* order, hipStream_t s) {
   if (blocks <= 0) return;
   if (al == FPV_CELL) { launch_forward_cell(d_tasks, ntasks, base, blocks, s); return; }
   const dim3 g(xcd_grid(blocks)), t(FPL_THREADS);
   if (al >= 4)
-    hipLaunchKernelGGL(fpl_kernel<4>, g, t, 0, s, d_tasks, ntasks, base, blocks);
+    hipLaunchKernelGGL(fpl_kernel<4>, g, t, 0, s, d_tasks, ntasks, base, blocks, order);
   else if (al == 2)
-    hipLaunchKernelGGL(fpl_kernel<2>, g, t, 0, s, d_tasks, ntasks, base, blocks);
+    hipLaunchKernelGGL(fpl_kernel<2>, g, t, 0, s, d_tasks, ntasks, base, blocks, order);
   else
-    hipLaunchKernelGGL(fpl_kernel<1>, g, t, 0, s, d_tasks, ntasks, base, blocks);
+    hipLaunchKernelGGL(fpl_kernel<1>, g, t, 0, s, d_tasks, ntasks, base, blocks, order);
+}
+
+namespace {
+// Length of the part of the line {d*n + t*u}, u = (cos a, sin a),
+// n = (-sin a, cos a), inside the square [-h, h]^2 (slab intersection).
+double square_chord(double d, double a, double h) {
+  const double ux = std::cos(a), uy = std::sin(a), px = -d * uy, py = d * ux;
+  double lo = -1e300, hi = 1e300;
+  for (int k = 0; k < 2; ++k) {
+    const double u = k ? uy : ux, p = k ? py : px;
+    if (std::fabs(u) < 1e-12) {
+      if (std::fabs(p) > h) return 0.0;
+      continue;
+    }
+    const double t0 = (-h - p) / u, t1 = (h - p) / u;
+    lo = std::max(lo, std::min(t0, t1));
+    hi = std::min(hi, std::max(t0, t1));
+  }
+  return std::max(0.0, hi - lo);
+}
+}  // namespace
+
+// A workgroup's time is about proportional to its rays' steps inside the
+// image, so a ray block's blocks are weighted by (rays) x (mean chord of its
+// middle ray through the valid cells, over 32 angles in [0, pi)).  Greedy
+// (longest processing time first): ray blocks by decreasing weight, each to
+// the share with the least work so far that has room (split over two shares
+// when it does not fit), so the shares carry about the same work and each
+// runs its longest workgroups first and its shortest last, where they
+// shorten the time the XCD's CUs wait for the last one.
+std::vector<int> fpl_dispatch_order(const std::vector<FplGeom>& g, int base, int n) {
+  std::vector<int> order((size_t)std::max(n, 0));
+  if (n <= 0) return order;
+  struct Seg { int first, count; double w; };
+  std::vector<Seg> segs;
+  for (const FplGeom& t : g) {
+    const int nrb = (t.N + 63) / 64;
+    const double m = 0.5 * (t.N + 1), h = 0.5 * (t.N - 2);
+    for (int rb = 0; rb < nrb; ++rb) {
+      const int lo = std::max(base, t.block_begin + rb * t.nab);
+      const int hi = std::min(base + n, t.block_begin + (rb + 1) * t.nab);
+      if (lo >= hi) continue;
+      const int rays = std::min(64, t.N - rb * 64);
+      const double d = (rb * 64 + 0.5 * (rays - 1) + 1.0) - m;
+      double chord = 0.0;
+      for (int k = 0; k < 32; ++k) chord += square_chord(d, (k + 0.5) * (3.141592653589793 / 32), h);
+      segs.push_back({lo - base, hi - lo, rays * chord / 32});
+    }
+  }
+  std::stable_sort(segs.begin(), segs.end(), [](const Seg& a, const Seg& b) { return a.w > b.w; });
+  const int C = xcd_grid(n) / XCDS;
+  std::vector<int> bins[XCDS];
+  double load[XCDS] = {};
+  int cap[XCDS];
+  for (int k = 0; k < XCDS; ++k) cap[k] = std::max(0, std::min(C, n - k * C));
+  for (const Seg& sg : segs) {
+    int first = sg.first, left = sg.count;
+    while (left > 0) {
+      int best = -1;
+      for (int k = 0; k < XCDS; ++k)
+        if ((int)bins[k].size() < cap[k] && (best < 0 || load[k] < load[best])) best = k;
+      if (best < 0) throw Error("fpl_dispatch_order: blocks outside the slices' ranges");
+      const int take = std::min(left, cap[best] - (int)bins[best].size());
+      for (int i = 0; i < take; ++i) bins[best].push_back(first + i);
+      load[best] += take * sg.w;
+      first += take;
+      left -= take;
+    }
+  }
+  size_t pos = 0;
+  for (int k = 0; k < XCDS; ++k) {
+    if ((int)bins[k].size() != cap[k]) throw Error("fpl_dispatch_order: the slices do not cover the launch");
+    for (int v : bins[k]) order[pos++] = v;
+  }
+  return order;
 }
 
 }  // namespace acu
