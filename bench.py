@@ -6,7 +6,7 @@ angles on [0, 180), a 2048^2 phantom of 40 seeded ellipses, Poisson noise at
 I0 = 5e3, Ram-Lak FBP warm start then split-Bregman TV with mu = 0.05,
 lambda = 0.1, 50 outer x 1 inner x 5 CG iterations -- FP32 operators, FP64
 CG scalars.  A step is that whole reconstruction for a batch of `--batch`
-slices (default 4, ~3.5 s) resident in HBM, run by the batched path (every
+slices (default 8, ~7.3 s) resident in HBM, run by the batched path (every
 kernel launch covers all slices, CG scalars stay on the device).  `value`
 is slices/s over the timed steps, with HIP events bracketing every operator
 launch on the library's stream (they give the roofline's kernel times); the
@@ -1029,7 +1029,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default: production 5; ingest 50; others 2)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=4, help="slices per step (production workload)")
+    ap.add_argument("--batch", type=int, default=8, help="slices per step (production workload)")
     ap.add_argument("--budget-s", type=float, default=400.0,
                     help="production: wall-clock budget from start; the legs after the timed steps "
                          "(FBP, plain paths, CPU) run only while it lasts (the timed work never shrinks)")
