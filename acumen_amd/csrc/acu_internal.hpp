@@ -197,8 +197,14 @@ void fft_tables(int N, const float* taps, float* G, float2* tw);
 void launch_ramp_fft(const FFTTask* d_tasks, int ntasks, int base, int blocks, int maxL, hipStream_t s);
 // refpos: positions in the reference's FP32 arithmetic (default); false:
 // tile-relative positions (closer to FP64, two VALU fewer per update)
+// XCD shares of a backprojection launch over `tiles` tiles: XCD k runs the
+// launch's tiles [b[k], b[k+1]) in order (b[XCDS] = tiles), at most C each.
+struct BPShares { int b[XCDS + 1]; int C; };
+// Shares of about equal work for tiles of estimated work w (launch order):
+// contiguous, so an XCD's L2 serves neighbouring tiles of one slice.
+BPShares bp_xcd_shares(const std::vector<float>& w);
 void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, int ty, bool refpos,
-                        hipStream_t s);
+                        const BPShares& sh, hipStream_t s);
 void launch_cells(const CellTask* d_tasks, int ntasks, int base, int blocks, hipStream_t s);
 // rowterm[t*N + y] = cos(c_t) * ((y+1) - mid), one FP32 product each (the
 // backprojector's row term of tv_hpc.F90:114), for t < T, y < N

@@ -174,6 +174,8 @@ struct Slice {
   std::vector<float> theta;
   bool theta_set = false;
   int ntx = 0, nty = 0, nsub = 0;  // backprojection tiles along x, y; 16-row sub-tiles along y
+  std::vector<uint8_t> tact_h;     // host copy of the sub-tile activity (tact)
+  BPShares bp_sh{};                // XCD shares of this slice's own backprojection launch
   // operator arena
   float *h = nullptr, *cmask = nullptr;
   float2 *scbp = nullptr, *scfp = nullptr;
@@ -259,6 +261,7 @@ struct acu_batch {
   bool fpl_order = true;
   int ramp_variant = acu::RAMP_FP64;  // arithmetic of the ramp filter
   int bp_ty = acu::BP_TY_DEFAULT;     // backprojection tile height (16, 32 or 64)
+  acu::BPShares bp_sh_all{};          // XCD shares of the backprojection over all slices
   bool bp_refpos = true;              // FBP backprojection positions in the reference's arithmetic
   bool tv_bp_refpos = true;           // the same inside tvopt's iteration
   // task sets for the fixed operand combinations of FBP / TV
@@ -395,6 +398,7 @@ void upload_tables(acu_batch* b, int i) {
   ACU_HIP_CHECK(hipMemcpyAsync(s.cmask, cm.data(), cm.size() * 4, hipMemcpyHostToDevice, st));
   ACU_HIP_CHECK(hipMemcpyAsync(s.tact, act.data(), act.size(), hipMemcpyHostToDevice, st));
   ACU_HIP_CHECK(hipStreamSynchronize(st));
+  s.tact_h = std::move(act);
 }
 
 // Tables of slice i from slice j of the same (N, T, wpu), device to device:
@@ -406,6 +410,7 @@ void copy_tables(acu_batch* b, int j, int i) {
   ACU_HIP_CHECK(hipMemcpyAsync(s.h, a.h, (size_t)(2 * s.N + 1) * 4, hipMemcpyDeviceToDevice, st));
   ACU_HIP_CHECK(hipMemcpyAsync(s.cmask, a.cmask, (size_t)s.N * s.N * 4, hipMemcpyDeviceToDevice, st));
   ACU_HIP_CHECK(hipMemcpyAsync(s.tact, a.tact, (size_t)s.ntx * s.nsub, hipMemcpyDeviceToDevice, st));
+  s.tact_h = a.tact_h;
   s.ff_edge = a.ff_edge;
   if (s.fftG) {
     const size_t L = (size_t)fft_len(s.N);
@@ -570,14 +575,46 @@ void refresh_fp(acu_batch* b) {
   b->drop_graphs();
 }
 
-// Backprojection tile ranges at the batch's tile height.
+// Estimated work of slice s's backprojection tiles at height ty, in launch
+// order: T for a tile with a mask pixel, 2% of that for one that only stores
+// zeros (every slice's tables must be built).
+void bp_tile_work(const Slice& s, int ty, std::vector<float>& w) {
+  const int nty = bp_tiles_y(s.N, ty), subs = ty / BP_SUB;
+  for (int y = 0; y < nty; ++y)
+    for (int x = 0; x < s.ntx; ++x) {
+      bool act = false;
+      for (int j = 0; j < subs; ++j) {
+        const int r = y * subs + j;
+        if (r < s.nsub) act = act || s.tact_h[(size_t)r * s.ntx + x] != 0;
+      }
+      w.push_back(act ? (float)s.T : 0.02f * (float)s.T);
+    }
+}
+
+// XCD shares of the all-slice and per-slice backprojection launches.
+void assign_bp_shares(acu_batch* b) {
+  std::vector<float> all, one;
+  for (auto& s : b->s) {
+    one.clear();
+    bp_tile_work(s, b->bp_ty, one);
+    s.bp_sh = bp_xcd_shares(one);
+    all.insert(all.end(), one.begin(), one.end());
+  }
+  b->bp_sh_all = bp_xcd_shares(all);
+}
+
+// Backprojection tile ranges at the batch's tile height (and their XCD
+// shares once the slices' tables exist).
 void assign_bp(acu_batch* b) {
   int bp = 0;
+  bool tables = true;
   for (auto& s : b->s) {
     s.nty = bp_tiles_y(s.N, b->bp_ty);
     s.bp_begin = bp; s.bp_n = s.ntx * s.nty; bp += s.bp_n;
+    tables = tables && !s.tact_h.empty();
   }
   b->bp_total = bp;
+  if (tables) assign_bp_shares(b);
 }
 
 void build_op_tasks(acu_batch* b);
@@ -656,6 +693,7 @@ void require_theta(acu_batch* b) {
 struct Range { int base, n; };
 Range rf_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->rf_total} : Range{b->s[sl].rf_begin, b->s[sl].rf_n}; }
 Range bp_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->bp_total} : Range{b->s[sl].bp_begin, b->s[sl].bp_n}; }
+const BPShares& bp_shares(const acu_batch* b, int sl) { return sl < 0 ? b->bp_sh_all : b->s[sl].bp_sh; }
 Range ew_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->ew_total} : Range{b->s[sl].ew_begin, b->s[sl].ew_n}; }
 Range sw_range(acu_batch* b, int sl) { return sl < 0 ? Range{0, b->sw_total} : Range{b->s[sl].sw_begin, b->s[sl].sw_n}; }
 
@@ -676,7 +714,7 @@ void iradon_seq(acu_batch* b, const RFSet& rf, const DevArray<BPTask>& bp, int s
   acu_ctx* c = b->ctx;
   const Range r2 = bp_range(b, sl);
   ramp_seq(b, rf, sl);
-  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, b->bp_ty, refpos, c->stream); });
+  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, b->bp_ty, refpos, bp_shares(b, sl), c->stream); });
   check_launch();
 }
 
@@ -796,7 +834,7 @@ void fbp_all(acu_batch* b, int sl) {
 void bp_seq(acu_batch* b, const DevArray<BPTask>& bp, int sl, bool refpos) {
   acu_ctx* c = b->ctx;
   const Range r2 = bp_range(b, sl);
-  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, b->bp_ty, refpos, c->stream); });
+  timed(c, K_BP, [&] { launch_backproject(bp.d, bp.n, r2.base, r2.n, b->bp_ty, refpos, bp_shares(b, sl), c->stream); });
   check_launch();
 }
 
@@ -968,7 +1006,7 @@ void bp_dev(acu_batch* b, int sl, float* img, const float* filt) {
   BPTask t = bp_task(s, filt, img);
   t.tile_begin = 0;
   BPTask* d = put_adhoc(b, t);
-  timed(c, K_BP, [&] { launch_backproject(d, 1, 0, s.bp_n, b->bp_ty, b->bp_refpos, c->stream); });
+  timed(c, K_BP, [&] { launch_backproject(d, 1, 0, s.bp_n, b->bp_ty, b->bp_refpos, s.bp_sh, c->stream); });
   check_launch();
 }
 
@@ -1132,7 +1170,10 @@ double time_bp(acu_batch* b, int ty, int reps) {
   }
   DevArray<BPTask> arr;
   arr.upload(tv, b->ctx->stream);
-  auto run = [&] { launch_backproject(arr.d, arr.n, 0, tiles, ty, b->bp_refpos, c->stream); };
+  std::vector<float> w;
+  for (int i = 0; i < ns; ++i) bp_tile_work(b->s[i], ty, w);
+  const BPShares sh = bp_xcd_shares(w);
+  auto run = [&] { launch_backproject(arr.d, arr.n, 0, tiles, ty, b->bp_refpos, sh, c->stream); };
   hipEvent_t e0 = c->ev(), e1 = c->ev();
   run();
   ACU_HIP_CHECK(hipEventRecord(e0, c->stream));
@@ -1490,6 +1531,7 @@ int acu_batch_create(acu_ctx* ctx, int nslices, const int* nimage, const int* nt
         acu::upload_theta(b.get(), i);
       }
     }
+    acu::assign_bp_shares(b.get());
     if (b->ramp_variant == acu::RAMP_FFT)
       for (auto& s : b->s)
         if (!s.fftG)

@@ -37,6 +37,8 @@
 //    zeros.
 #include "acu_internal.hpp"
 
+#include <algorithm>
+
 namespace acu {
 namespace {
 
@@ -60,7 +62,8 @@ __device__ __forceinline__ int find_task(const BPTask* tasks, int ntasks, int gt
 }
 
 template <int TY, bool REFPOS>
-__global__ __launch_bounds__(64 * bp_waves(TY)) void bp_kernel(const BPTask* __restrict__ tasks, int ntasks, int base) {
+__global__ __launch_bounds__(64 * bp_waves(TY)) void bp_kernel(const BPTask* __restrict__ tasks, int ntasks, int base,
+                                                               BPShares sh) {
   constexpr int NW = bp_waves(TY), NT = 64 * NW, PY = TY / NW, W = bp_window(TY), SUBS = TY / BP_SUB;
   constexpr int SPA = NT / AB, NST = W / SPA;  // staging threads per angle, pairs per thread
   static_assert(TY % BP_SUB == 0 && TY % NW == 0 && NT % AB == 0 && W % SPA == 0, "tile shape");
@@ -68,7 +71,12 @@ __global__ __launch_bounds__(64 * bp_waves(TY)) void bp_kernel(const BPTask* __r
   __shared__ int lo_s[AB];
   __shared__ float k_s[AB];    // K of the tile-local position (see above)
 
-  const int gtile = base + blockIdx.x;
+  // XCD k (xcd_remap) runs the tiles of its share in order
+  const int l = xcd_remap(blockIdx.x, gridDim.x);
+  const int k = l / sh.C;
+  const int lt = sh.b[k] + (l - k * sh.C);
+  if (lt >= sh.b[k + 1]) return;
+  const int gtile = base + lt;
   const BPTask& tk = tasks[find_task(tasks, ntasks, gtile)];
   const int N = tk.N, T = tk.T;
   const float m = tk.mid;
@@ -221,19 +229,45 @@ int bp_tiles_x(int N) { return (N + TX - 1) / TX; }
 int bp_tiles_y(int N, int ty) { return (N + ty - 1) / ty; }
 
 template <bool REFPOS>
-void launch_bp(const BPTask* d_tasks, int ntasks, int base, int tiles, int ty, hipStream_t s) {
+void launch_bp(const BPTask* d_tasks, int ntasks, int base, int ty, const BPShares& sh, hipStream_t s) {
+  const dim3 g((unsigned)(XCDS * sh.C));
   switch (ty) {
-    case 16: hipLaunchKernelGGL((bp_kernel<16, REFPOS>), dim3(tiles), dim3(64 * bp_waves(16)), 0, s, d_tasks, ntasks, base); break;
-    case 32: hipLaunchKernelGGL((bp_kernel<32, REFPOS>), dim3(tiles), dim3(64 * bp_waves(32)), 0, s, d_tasks, ntasks, base); break;
-    case 64: hipLaunchKernelGGL((bp_kernel<64, REFPOS>), dim3(tiles), dim3(64 * bp_waves(64)), 0, s, d_tasks, ntasks, base); break;
+    case 16: hipLaunchKernelGGL((bp_kernel<16, REFPOS>), g, dim3(64 * bp_waves(16)), 0, s, d_tasks, ntasks, base, sh); break;
+    case 32: hipLaunchKernelGGL((bp_kernel<32, REFPOS>), g, dim3(64 * bp_waves(32)), 0, s, d_tasks, ntasks, base, sh); break;
+    case 64: hipLaunchKernelGGL((bp_kernel<64, REFPOS>), g, dim3(64 * bp_waves(64)), 0, s, d_tasks, ntasks, base, sh); break;
     default: throw Error("backprojection tile height must be 16, 32 or 64");
   }
 }
 
-void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, int ty, bool refpos, hipStream_t s) {
+void launch_backproject(const BPTask* d_tasks, int ntasks, int base, int tiles, int ty, bool refpos,
+                        const BPShares& sh, hipStream_t s) {
   if (tiles <= 0) return;
-  if (refpos) launch_bp<true>(d_tasks, ntasks, base, tiles, ty, s);
-  else launch_bp<false>(d_tasks, ntasks, base, tiles, ty, s);
+  if (sh.b[0] != 0 || sh.b[XCDS] != tiles || sh.C < 1) throw Error("backprojection shares do not match the launch");
+  if (refpos) launch_bp<true>(d_tasks, ntasks, base, ty, sh, s);
+  else launch_bp<false>(d_tasks, ntasks, base, ty, sh, s);
+}
+
+// Cut points at k/XCDS of the total estimated work (a tile starts the next
+// share once the work before it reaches the share's target), so the shares
+// are contiguous runs of the launch's tiles with about equal work; with 8
+// equal slices each XCD gets one slice.
+BPShares bp_xcd_shares(const std::vector<float>& w) {
+  BPShares sh{};
+  const int n = (int)w.size();
+  double tot = 0.0;
+  for (float v : w) tot += v;
+  int k = 1;
+  double acc = 0.0;
+  for (int i = 0; i < n && k < XCDS; ++i) {
+    while (k < XCDS && acc >= tot * k / XCDS - 1e-9 * tot) sh.b[k++] = i;
+    acc += w[i];
+  }
+  while (k < XCDS) sh.b[k++] = n;
+  sh.b[0] = 0;
+  sh.b[XCDS] = n;
+  sh.C = 1;
+  for (int j = 0; j < XCDS; ++j) sh.C = std::max(sh.C, sh.b[j + 1] - sh.b[j]);
+  return sh;
 }
 
 }  // namespace acu

@@ -1,9 +1,12 @@
 """The LDS projector's dispatch order (fpl_dispatch_order in
-acumen_amd/csrc/fpl.hip), checked on the CPU through the built library: for
-single slices, the 4-slice 2049^2 x 1500 launch and random mixed batches it
-is a permutation of the launch's blocks (a wrong index would drop or repeat
-ray blocks), and the angle blocks of a ray block stay consecutive and in
-order within an XCD's share."""
+acumen_amd/csrc/fpl.hip) and the backprojector's XCD shares (bp_xcd_shares in
+bp.hip), checked on the CPU through the built library.  The order, for single
+slices, the 4-slice 2049^2 x 1500 launch and random mixed batches, is a
+permutation of the launch's blocks (a wrong index would drop or repeat ray
+blocks), and the angle blocks of a ray block stay consecutive and in order
+within an XCD's share.  The shares cover the launch's tiles in order, each
+within one tile's work of an eighth of the total, and 8 equal slices give
+one slice per XCD."""
 import os
 import shutil
 import subprocess
@@ -21,6 +24,24 @@ CHECKER = r"""
 namespace acu {
 struct FplGeom { int N, nab, block_begin; };
 std::vector<int> fpl_dispatch_order(const std::vector<FplGeom>& g, int base, int n);
+struct BPShares { int b[9]; int C; };
+BPShares bp_xcd_shares(const std::vector<float>& w);
+}
+static int check_shares(const std::vector<float>& w) {
+  const acu::BPShares sh = acu::bp_xcd_shares(w);
+  const int n = (int)w.size();
+  if (sh.b[0] != 0 || sh.b[8] != n) return 11;
+  double tot = 0, mx = 0;
+  for (float v : w) { tot += v; mx = std::max(mx, (double)v); }
+  int C = 1;
+  for (int k = 0; k < 8; ++k) {
+    if (sh.b[k + 1] < sh.b[k]) return 12;
+    C = std::max(C, sh.b[k + 1] - sh.b[k]);
+    double s = 0;
+    for (int i = sh.b[k]; i < sh.b[k + 1]; ++i) s += w[i];
+    if (s > tot / 8 + mx + 1e-6 * tot) return 13;
+  }
+  return C == sh.C ? 0 : 14;
 }
 using acu::FplGeom;
 static int blocks(const FplGeom& t) { return (t.N + 63) / 64 * t.nab; }
@@ -61,7 +82,21 @@ int main() {
   if (int r = check(g, 0, b)) { printf("headline %d\n", r); return 1; }
   for (int i = 0; i < 4; ++i)
     if (int r = check({g[i]}, g[i].block_begin, blocks(g[i]))) { printf("slice %d: %d\n", i, r); return 1; }
+  {
+    std::vector<float> w;
+    for (int i = 0; i < 8 * 2145; ++i) w.push_back((i % 2145) < 300 ? 30.0f : 1500.0f);
+    const acu::BPShares sh = acu::bp_xcd_shares(w);
+    for (int k = 0; k <= 8; ++k) if (sh.b[k] != 2145 * k) { printf("shares of 8 slices\n"); return 1; }
+    if (int r = check_shares(w)) { printf("shares %d\n", r); return 1; }
+  }
   std::mt19937 rng(11);
+  for (int n = 1; n <= 20; ++n)
+    if (int r = check_shares(std::vector<float>(n, 7.0f))) { printf("small shares %d: %d\n", n, r); return 1; }
+  for (int it = 0; it < 500; ++it) {
+    std::vector<float> w(1 + rng() % 20000);
+    for (auto& v : w) v = (rng() % 4 == 0) ? 1.0f : (float)(360 + rng() % 1141);
+    if (int r = check_shares(w)) { printf("random shares %d: %d\n", it, r); return 1; }
+  }
   for (int it = 0; it < 2000; ++it) {
     std::vector<FplGeom> m;
     int bb = 0;
