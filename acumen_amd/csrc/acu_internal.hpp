@@ -233,11 +233,11 @@ int fpl_blocks(int N, int T, int al);
 // block (blocks rb * nab + ab from there, rb = ray block of 64 rays).
 struct FplGeom { int N, nab, block_begin; };
 // Dispatch order of a launch over blocks [base, base + n) of the slices g:
-// a permutation of [0, n) that gives each XCD's share of the grid (the
+// empty when the natural order gives each XCD's share of the grid (the
 // logical blocks [k*C, (k+1)*C), C = xcd_grid(n) / XCDS, that xcd_remap
-// hands XCD k) about the same estimated work, longest workgroups first
-// (shortest first in launches of 6 or more workgroups per slot).  A ray
-// block's angle blocks stay consecutive.
+// hands XCD k) about the same estimated work, else a permutation of [0, n)
+// that does, longest workgroups first (shortest first in launches of 6 or
+// more workgroups per slot).  A ray block's angle blocks stay consecutive.
 std::vector<int> fpl_dispatch_order(const std::vector<FplGeom>& g, int base, int n);
 int fplc_blocks(int N, int T);
 // angles per workgroup block of an LDS projector variant

@@ -524,7 +524,10 @@ void assign_fp(acu_batch* b) {
       const FplGeom t{s.N, (s.T + fpl_angle_block(s.fpv) - 1) / fpl_angle_block(s.fpv), s.fl_begin};
       geo.push_back(t);
       const std::vector<int> o = fpl_dispatch_order({t}, s.fl_begin, s.fl_n);
-      std::copy(o.begin(), o.end(), one.begin() + s.fl_begin);
+      if (o.empty())  // the natural order
+        for (int i = 0; i < s.fl_n; ++i) one[(size_t)s.fl_begin + i] = i;
+      else
+        std::copy(o.begin(), o.end(), one.begin() + s.fl_begin);
     }
     b->fl_ord_all[g].upload(fpl_dispatch_order(geo, 0, fl[g]), b->ctx->stream);
     b->fl_ord_one[g].upload(one, b->ctx->stream);

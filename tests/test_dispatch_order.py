@@ -1,8 +1,9 @@
 """The LDS projector's dispatch order (fpl_dispatch_order in
 acumen_amd/csrc/fpl.hip) and the backprojector's XCD shares (bp_xcd_shares in
 bp.hip), checked on the CPU through the built library.  The order, for single
-slices, the 4-slice 2049^2 x 1500 launch and random mixed batches, is a
-permutation of the launch's blocks (a wrong index would drop or repeat ray
+slices, the 4-slice 2049^2 x 1500 launch and random mixed batches, is empty
+(the natural order, kept when its XCD shares are balanced, as with 8 equal
+slices) or a permutation of the launch's blocks (a wrong index would drop or repeat ray
 blocks), and the angle blocks of a ray block stay consecutive and in order
 within an XCD's share.  The shares cover the launch's tiles in order, each
 within one tile's work of an eighth of the total, and 8 equal slices give
@@ -47,7 +48,9 @@ using acu::FplGeom;
 static int blocks(const FplGeom& t) { return (t.N + 63) / 64 * t.nab; }
 // the launch [base, base + n) covers whole slices g
 static int check(const std::vector<FplGeom>& g, int base, int n) {
-  const std::vector<int> o = acu::fpl_dispatch_order(g, base, n);
+  std::vector<int> o = acu::fpl_dispatch_order(g, base, n);
+  if (o.empty())  // the natural order
+    for (int i = 0; i < n; ++i) o.push_back(i);
   if ((int)o.size() != n) return 1;
   std::vector<int> seen(n, 0);
   for (int v : o) {
@@ -80,6 +83,14 @@ int main() {
   int b = 0;
   for (int i = 0; i < 4; ++i) { g.push_back({2049, 94, b}); b += blocks(g.back()); }
   if (int r = check(g, 0, b)) { printf("headline %d\n", r); return 1; }
+  // 8 equal slices: one slice per XCD in the natural order, kept; one slice: reordered
+  {
+    std::vector<FplGeom> g8;
+    int b8 = 0;
+    for (int i = 0; i < 8; ++i) { g8.push_back({2049, 94, b8}); b8 += blocks(g8.back()); }
+    if (!acu::fpl_dispatch_order(g8, 0, b8).empty()) { printf("8 slices reordered\n"); return 1; }
+    if (acu::fpl_dispatch_order({g[0]}, 0, blocks(g[0])).empty()) { printf("1 slice kept\n"); return 1; }
+  }
   for (int i = 0; i < 4; ++i)
     if (int r = check({g[i]}, g[i].block_begin, blocks(g[i]))) { printf("slice %d: %d\n", i, r); return 1; }
   {
