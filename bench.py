@@ -21,7 +21,8 @@ reference's 7 radon + 7 iradon per outer iteration; the CPU baseline.
 roofline: the forward projector dominates TV time; its algorithmic work is
 14 FP32 flops per ray sample (2 coordinates, 2 fractions, 3 lerps, 1
 accumulate) over N*N*T samples per slice.  It runs on the FP32 VALU with
-its bilinear gathers from LDS, so `bound` is "valu", `frac` is against the
+its bilinear gathers from LDS, whose bank conflicts bound it, so `bound` is
+"lds" (the `lds` object gives its LDS read rate), `frac` is against the
 published FP32 vector peak (157.3 TFLOP/s, packed issue) and
 `frac_nonpacked` against the 78.6 TFLOP/s of one FMA per lane and clock,
 the rate of code built without packed FP32.  `traffic` is the HBM bytes of
@@ -936,9 +937,11 @@ def run_production(args) -> int:
     traffic = traffic_from_profile(traffic_file)
     lds_bytes = LDS_BYTES_PER_SAMPLE * samples_per_launch / (fp_avg_ms * 1e-3)
     roofline = {
-        "bound": "valu",
+        "bound": "lds",
         "kernel": f"{fp_name} (ray-driven bilinear projection on the FP32 VALU, gathers from LDS)",
-        "limiters": "VALU issue and LDS bank conflicts of the bilinear gathers (SQ counters under profiles/)",
+        "limiters": "bank conflicts of the bilinear ds_read2 gathers (LDS busy ~55% of the cycles, three "
+                    "quarters of it in conflicts) beside 0.96 VALU wave-instructions per CU and cycle "
+                    "(SQ counters under profiles/)",
         "achieved": round(achieved, 3),
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
@@ -954,7 +957,8 @@ def run_production(args) -> int:
         "launches": fp_n,
         "work_per_launch": f"{FP_FLOPS_PER_SAMPLE} flop x {N}*{N}*{T} samples x {B} slices",
         "lds": {"bytes_per_sample": LDS_BYTES_PER_SAMPLE, "achieved_TBps": round(lds_bytes / 1e12, 2),
-                "B_per_clk_per_cu": round(lds_bytes / (N_CU * CLOCK_GHZ * 1e9), 1)},
+                "B_per_clk_per_cu": round(lds_bytes / (N_CU * CLOCK_GHZ * 1e9), 1),
+                "frac_of_128_B_per_clk": round(lds_bytes / (N_CU * CLOCK_GHZ * 1e9) / 128.0, 3)},
     }
 
     bp_ms, bp_n = kt["bp"]
