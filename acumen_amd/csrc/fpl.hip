@@ -203,26 +203,54 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
         bxl[j] = act[j] ? (m - sy[j]) - (float)bx0 : 1.0f;
         byl[j] = act[j] ? (m + cyv[j]) - (float)by0 : 1.0f;
       }
-      // u = x - m is an integer below 2^24 in magnitude, so stepping it by
-      // 1.0f gives exactly (float)x - m without a conversion per step
-      float u = (float)c0 - m;
-#pragma unroll 2
-      for (int x = c0; x <= c1; ++x, u += 1.0f) {
-        float val[AL];
+      // Software-pipelined over two groups of the lane's angles: a group's
+      // taps for the next step are read from LDS right after its lerps of
+      // this step, so they are in flight while the other group's lerps run
+      // (a scheduling barrier keeps the compiler from merging the groups
+      // back into one wait).  Each angle still adds its samples in step
+      // order.  u = x - m is an integer below 2^24 in magnitude, so
+      // stepping it by 1.0f gives exactly (float)x - m without a conversion.
+      constexpr int NG = AL < 2 ? AL : 2;  // angle groups
+      constexpr int H = AL / NG;           // angles per group
+      struct Taps { float fx, fy, v00, v10, v01, v11; };
+      Taps tp[NG][H];
+      auto fetch = [&](float uu, int g) {
 #pragma unroll
-        for (int j = 0; j < AL; ++j) {
-          const float xl = fmaf(cj[j], u, bxl[j]);
-          const float yl = fmaf(sj[j], u, byl[j]);
-          const float fx = __builtin_amdgcn_fractf(xl), fy = __builtin_amdgcn_fractf(yl);
+        for (int q = 0; q < H; ++q) {
+          const int j = g * H + q;
+          const float xl = fmaf(cj[j], uu, bxl[j]);
+          const float yl = fmaf(sj[j], uu, byl[j]);
+          tp[g][q].fx = __builtin_amdgcn_fractf(xl);
+          tp[g][q].fy = __builtin_amdgcn_fractf(yl);
           const float* p0 = (const float*)((const char*)box + fpl_cell_offset<PITCH>(xl, yl));
-          const float v00 = p0[0], v10 = p0[1], v01 = p0[PITCH], v11 = p0[PITCH + 1];
-          const float top = fmaf(fx, v10 - v00, v00);
-          const float bot = fmaf(fx, v11 - v01, v01);
-          val[j] = fmaf(fy, bot - top, top);
+          tp[g][q].v00 = p0[0];
+          tp[g][q].v10 = p0[1];
+          tp[g][q].v01 = p0[PITCH];
+          tp[g][q].v11 = p0[PITCH + 1];
         }
+      };
+      auto sample = [&](int g) {
 #pragma unroll
-        for (int j = 0; j < AL; ++j) acc[j] += val[j];
+        for (int q = 0; q < H; ++q) {
+          const Taps& t = tp[g][q];
+          const float top = fmaf(t.fx, t.v10 - t.v00, t.v00);
+          const float bot = fmaf(t.fx, t.v11 - t.v01, t.v01);
+          acc[g * H + q] += fmaf(t.fy, bot - top, top);
+        }
+      };
+      float u = (float)c0 - m;
+#pragma unroll
+      for (int g = 0; g < NG; ++g) fetch(u, g);
+      for (int x = c0; x < c1; ++x, u += 1.0f) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+          sample(g);
+          fetch(u + 1.0f, g);
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
+#pragma unroll
+      for (int g = 0; g < NG; ++g) sample(g);  // step c1
     } else {
       // border chunk: the reference's positions and bounds test
       for (int x = c0; x <= c1; ++x) {
