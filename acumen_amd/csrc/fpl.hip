@@ -11,16 +11,19 @@
 // Every lane sums its rays' samples in step order, so the per-ray summation
 // order is the reference's.  The sweep goes in chunks of FPL_U steps; for a
 // chunk the workgroup stages in LDS the bounding box of all its samples
-// (raw FP32 pixels, zero outside the image, fixed row pitch FPL_PITCH so
-// both rows of a bilinear cell come from ds_read2 immediates) and every
-// angle of the block
-// reads it, so one staged pixel serves ~2-3 samples and the gathers never
-// leave the CU (the global-gather kernel of fp.hip moves ~23 GB per
-// 2049^2 x 1500 launch; this one stages ~1.7 bytes per sample from L2).
+// (raw FP32 pixels, zero outside the image, a row pitch per angle block
+// from a few odd pitches, so both rows of a bilinear cell come from
+// ds_read2 immediates and the rays' taps spread over the banks) and every
+// angle of the block reads it, so one staged pixel serves ~2-3 samples and
+// the gathers never leave the CU (the global-gather kernel of fp.hip moves
+// ~23 GB per 2049^2 x 1500 launch; this one stages ~1.7 bytes per sample
+// from L2).  The host orders the workgroups so that each XCD gets about the
+// same work (fpl_dispatch_order).
 //
 //  * interior chunks (every cell of the box passes the bounds test):
 //    positions are FMAs from the ray's start, there is no test, and a sample
-//    is 2 FMA + 2 cvt + 2 fract + addressing + 2 ds_read2_b32 + 3 lerps;
+//    is 2 FMA + 2 cvt + 2 fract + addressing + 2 ds_read2_b32 + 3 lerps,
+//    the steps software-pipelined over two groups of the lane's angles;
 //  * border chunks: positions are the reference's FP32 values (c*u, s*u,
 //    s*ym, c*ym rounded once, summed in its order, no contraction) and the
 //    bounds test is applied, so every edge decision is the reference's own;
