@@ -177,3 +177,41 @@ def test_autotune_report_and_results(ctx):
             assert rel(b.get_image(i, ACU_IMG_TV), tv64) < TV_TOL
         with pytest.raises(AcumenError):
             b.set_fp_variant(0, 3)
+
+
+@pytest.mark.parametrize("variant", [1, 2, 4])
+def test_dispatch_order_changes_no_bit(ctx, monkeypatch, variant):
+    """The LDS projector's dispatch orders (the work-balanced permutation of
+    a one-slice or mixed launch, the natural order of balanced launches)
+    only reorder workgroups: a mixed batch's TV and a single slice's radon
+    equal, bit for bit, those of batches created with ACUMEN_FPL_ORDER=rb
+    (every launch in ray-block order)."""
+    shapes = [(513, 360), (257, 180), (769, 240), (513, 360)]
+    say("dispatch order", shapes, variant)
+    rng = np.random.default_rng(23)
+    items = []
+    for i, (N, T) in enumerate(shapes):
+        th = np.sort(rng.uniform(0, 180, T)).astype(np.float32)
+        f = ref.radon(disk_phantom(N, 5, 60 + i), th, "f64").astype(np.float32)
+        items.append((N, T, synth.default_wpu(N), th, f))
+
+    def run():
+        with Batch(ctx, [(N, T, w, th) for N, T, w, th, _ in items]) as b:
+            for i, it in enumerate(items):
+                b.set_sino(i, it[4])
+                b.set_fp_variant(i, variant)
+            b.tvopt(0.05, 0.1, 2, 1, 3, ACU_TV_GRAPH)
+            tv = [b.get_image(i, ACU_IMG_TV) for i in range(len(items))]
+        N, T, w, th, _ = items[2]
+        with Batch(ctx, [(N, T, w, th)]) as p:
+            p.set_fp_variant(0, variant)
+            sino = p.radon(disk_phantom(N, 7, 99))
+        return tv, sino
+
+    monkeypatch.delenv("ACUMEN_FPL_ORDER", raising=False)
+    tv_bal, sino_bal = run()
+    monkeypatch.setenv("ACUMEN_FPL_ORDER", "rb")
+    tv_rb, sino_rb = run()
+    for i in range(len(items)):
+        assert np.array_equal(tv_bal[i].view(np.uint32), tv_rb[i].view(np.uint32)), i
+    assert np.array_equal(sino_bal.view(np.uint32), sino_rb.view(np.uint32))
