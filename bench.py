@@ -940,7 +940,7 @@ def run_production(args) -> int:
         "bound": "lds",
         "kernel": f"{fp_name} (ray-driven bilinear projection on the FP32 VALU, gathers from LDS)",
         "limiters": "bank conflicts of the bilinear ds_read2 gathers (LDS busy ~55% of the cycles, three "
-                    "quarters of it in conflicts) beside 0.96 VALU wave-instructions per CU and cycle "
+                    "quarters of it in conflicts) beside 0.98 VALU wave-instructions per CU and cycle "
                     "(SQ counters under profiles/)",
         "achieved": round(achieved, 3),
         "peak": FP32_PEAK_TFLOPS,
