@@ -247,8 +247,8 @@ inline int fpl_angle_block(int al) { return al == FPV_CELL ? 16 : 4 * al; }
 bool fpl_may_overflow(int N, int T, const float* theta_deg, int al);
 bool fplc_may_overflow(int N, int T, const float* theta_deg);
 // Pitch class of every block of 4*al angles: the pitch with the fewest LDS
-// bank conflicts for the block's angle among those whose rows hold the
-// block's box (out: ceil(T / (4*al)) bytes)
+// bank conflicts for the block's angles under the kernel's lane map, among
+// those whose rows hold the block's box (out: ceil(T / (4*al)) bytes)
 void fpl_pitch_classes(int N, int T, const float* theta_deg, int al, uint8_t* out);
 int cell_blocks(int N);
 

@@ -6,8 +6,11 @@
 //   xr = cos(c)*(x-m) - sin(c)*(y-m) + m,  yr = sin(c)*(x-m) + cos(c)*(y-m) + m,
 //   a sample counts only if its floor cell (x1, y1) has 2 <= x1, y1 <= N-1.
 //
-// gfx950 mapping.  One workgroup = 64 neighbouring rays (one lane each) x a
-// block of 4*al neighbouring angles (al per lane, one wave per al angles).
+// gfx950 mapping.  One workgroup = 64 neighbouring rays x a block of 4*al
+// neighbouring angles; a lane takes one ray and al angles, a wave 16 rays x
+// 4 consecutive angles per instruction (angles g, g+4, ... of the block for
+// lane group g), so a half-wave's LDS reads are 16 rays at two adjacent
+// angles, whose taps often share dwords, instead of 32 rays at one angle.
 // Every lane sums its rays' samples in step order, so the per-ray summation
 // order is the reference's.  The sweep goes in chunks of FPL_U steps; for a
 // chunk the workgroup stages in LDS the bounding box of all its samples
@@ -63,9 +66,9 @@ __device__ __forceinline__ unsigned fpl_cell_offset(float xl, float yl) {
   return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, pk), w, 0u, false);
 }
 
-// One workgroup's work; PITCH is the LDS row pitch of its boxes (odd, so
-// that the taps of 64 neighbouring rays spread over the banks; which odd
-// value spreads them best depends on the angle, see fpl_pitch_classes).
+// One workgroup's work; PITCH is the LDS row pitch of its boxes (which of
+// a few pitches spreads the lanes' taps best over the banks depends on the
+// angles, see fpl_pitch_classes).
 template <int AL, int PITCH>
 __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __restrict__ box, float (*red)[4]) {
 #pragma clang fp contract(off)
@@ -75,7 +78,10 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
   const int local = gb - tk.block_begin;
   const int rb = local / tk.nab, ab = local - rb * tk.nab;  // ray-block-major
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ray = rb * 64 + lane;
+  // wave w: rays 16w..16w+15 (lane & 15) of the block; lane group g = lane >> 4
+  // takes angles g, g + 4, ... (so the 64 lanes of one instruction cover 16 rays
+  // at 4 consecutive angles)
+  const int ray = rb * 64 + wave * 16 + (lane & 15);
   const float ym = (float)(min(ray, N - 1) + 1) - m;
 
   float s[AL], c[AL], sy[AL], cyv[AL], acc[AL];
@@ -84,7 +90,7 @@ __device__ __forceinline__ void fpl_body(const FPLTask& tk, int gb, float* __res
   float lu0 = 1e30f, lu1 = -1e30f;
 #pragma unroll
   for (int j = 0; j < AL; ++j) {
-    ang[j] = ab * 4 * AL + wave * AL + j;
+    ang[j] = ab * 4 * AL + (lane >> 4) + 4 * j;
     act[j] = ang[j] < T && ray < N;
     const float2 v = const_f2(tk.sc, min(ang[j], T - 1));
     s[j] = v.x;
@@ -368,42 +374,58 @@ int fpl_angles_per_lane(int N, int T) {
   while (al < AMAX && (4.0 * (2 * al) - 1.0) * dth * r <= room) al *= 2;
   return al;
 }
-// Bank cost of the two ds_read2 of one sample for 64 neighbouring rays
-// (one per lane) at projector angle (s, c), summed over a few sub-pixel
-// offsets.  Each ds_read2_b32 is costed as two passes (its first dword, then
-// its second) of each half-wave: per pass, the largest number of distinct
-// dwords one of the 32 banks serves.  Timed on gfx950 for 180 angles x 8
-// pitches, this cost orders the pitches as the measured read time does
-// (correlation 1.00 per angle); counting both dwords of a lane in one pass
-// picks pitches 5% slower on average.
-static int pitch_cost(int P, double s, double c) {
-  static const double off[4][2] = {{0.13, 0.71}, {0.57, 0.29}, {0.91, 0.47}, {0.35, 0.05}};
+// Bank cost of the interior loop's LDS reads for angle block [a0, a1] at
+// row pitch P: the two ds_read2_b32 of each of the al instructions of one
+// wave (16 neighbouring rays x 4 consecutive angles, the kernel's lane map),
+// at two fixed positions of the image.  Each ds_read2_b32 is costed as two
+// passes (its first dword, then its second) of each half-wave: per pass, the
+// largest number of distinct dwords one of the 32 banks serves (lanes that
+// read the same dword share it).  Timed on gfx950 for 180 angles x 8 pitches
+// with one angle per instruction, this per-pass cost orders the pitches as
+// the measured read time does (correlation 1.00 per angle).
+static int pitch_cost(int P, int N, const float* theta_deg, int a0, int a1, int al) {
+  static const double pos[2][2] = {{0.31, 0.23}, {-0.52, 0.37}};  // (step, ray) / (N/2) from the centre
+  const double pi = 3.141592653589793, h = 0.5 * N, m = 0.5 * (N + 1);
+  double sn[16], cs[16];
+  for (int k = 0; k < 4 * al; ++k) {
+    const double th = pi * ((double)theta_deg[std::min(a1, a0 + k)] - 90.0) / 180.0;
+    sn[k] = std::sin(th);
+    cs[k] = std::cos(th);
+  }
   int cost = 0;
-  for (const auto& o : off) {
-    int addr[64];
-    for (int k = 0; k < 64; ++k) {
-      const double x = 50.0 + o[0] - s * k, y = 50.0 + o[1] + c * k;
-      addr[k] = (int)std::floor(y) * P + (int)std::floor(x);
-    }
-    for (int rd = 0; rd < 2; ++rd)
-      for (int e = 0; e < 2; ++e)
-        for (int h = 0; h < 2; ++h) {
-          int d[32], nd = 0;
-          for (int k = 32 * h; k < 32 * h + 32; ++k) {
-            const int a = addr[k] + rd * P + e;
-            bool seen = false;
-            for (int q = 0; q < nd; ++q) seen = seen || d[q] == a;
-            if (!seen) d[nd++] = a;
+  for (const auto& q : pos) {
+    const double u = q[0] * h;
+    const int r0 = (int)std::lround(q[1] * h + m) - 8;
+    for (int j = 0; j < al; ++j) {
+      int addr[64];
+      for (int l = 0; l < 64; ++l) {
+        const int k = (l >> 4) + 4 * j;
+        const double ym = (double)(r0 + (l & 15) + 1) - m;
+        // + 4096: positive addresses, the same banks up to a rotation
+        const double x = cs[k] * u - sn[k] * ym + m + 4096.0, y = sn[k] * u + cs[k] * ym + m + 4096.0;
+        addr[l] = (int)std::floor(y) * P + (int)std::floor(x);
+      }
+      for (int rd = 0; rd < 2; ++rd)
+        for (int e = 0; e < 2; ++e)
+          for (int hw = 0; hw < 2; ++hw) {
+            int d[32], nd = 0;
+            for (int k = 32 * hw; k < 32 * hw + 32; ++k) {
+              const int a = addr[k] + rd * P + e;
+              bool seen = false;
+              for (int t = 0; t < nd; ++t) seen = seen || d[t] == a;
+              if (!seen) d[nd++] = a;
+            }
+            int cnt[32] = {0}, mx = 0;
+            for (int t = 0; t < nd; ++t) mx = std::max(mx, ++cnt[d[t] % 32]);
+            cost += mx;
           }
-          int cnt[32] = {0}, mx = 0;
-          for (int q = 0; q < nd; ++q) mx = std::max(mx, ++cnt[((d[q] % 32) + 32) % 32]);
-          cost += mx;
-        }
+    }
   }
   return cost;
 }
 
 void fpl_pitch_classes(int N, int T, const float* theta_deg, int al, uint8_t* out) {
+  if (al != 1 && al != 2 && al != 4) throw Error("fpl_pitch_classes: al must be 1, 2 or 4");
   const double pi = 3.141592653589793;
   const double r = 0.70711 * ((double)N + 2.0);
   const int ab = 4 * al;
@@ -425,14 +447,12 @@ void fpl_pitch_classes(int N, int T, const float* theta_deg, int al, uint8_t* ou
     const double bx = 63.0 * ms + (FPL_U - 1) * mc + 8.0, by = 63.0 * mc + (FPL_U - 1) * ms + 8.0;
     auto tolerance = [&](int P) { return std::min((double)P - bx, (double)(FPL_CAP / P) - by); };
     const double base_tol = std::min(tolerance(FPL_PITCHES[FPL_PCLASS_DEFAULT]), drift);
-    const double mid = pi * (0.5 * ((double)theta_deg[a0] + (double)theta_deg[a1]) - 90.0) / 180.0;
-    const double s = std::sin(mid), c = std::cos(mid);
-    int best = FPL_PCLASS_DEFAULT, best_cost = pitch_cost(FPL_PITCHES[FPL_PCLASS_DEFAULT], s, c);
+    int best = FPL_PCLASS_DEFAULT, best_cost = pitch_cost(FPL_PITCHES[FPL_PCLASS_DEFAULT], N, theta_deg, a0, a1, al);
     for (int k = 0; k < FPL_NPITCH; ++k) {
       const int P = FPL_PITCHES[k];
       // no more overflowing chunks than the default pitch
       if (k == FPL_PCLASS_DEFAULT || tolerance(P) < base_tol) continue;
-      const int cost = pitch_cost(P, s, c);
+      const int cost = pitch_cost(P, N, theta_deg, a0, a1, al);
       if (cost < best_cost) { best = k; best_cost = cost; }
     }
     out[blk] = (uint8_t)best;
