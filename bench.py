@@ -939,8 +939,8 @@ def run_production(args) -> int:
     roofline = {
         "bound": "lds",
         "kernel": f"{fp_name} (ray-driven bilinear projection on the FP32 VALU, gathers from LDS)",
-        "limiters": "bank conflicts of the bilinear ds_read2 gathers (LDS busy ~55% of the cycles, three "
-                    "quarters of it in conflicts) beside 0.98 VALU wave-instructions per CU and cycle "
+        "limiters": "bank conflicts of the bilinear ds_read2 gathers (LDS busy ~52% of the cycles, 70% "
+                    "of it in conflicts) beside 1.05 VALU wave-instructions per CU and cycle "
                     "(SQ counters under profiles/)",
         "achieved": round(achieved, 3),
         "peak": FP32_PEAK_TFLOPS,
