@@ -7,7 +7,8 @@ slices) or a permutation of the launch's blocks (a wrong index would drop or rep
 blocks), and the angle blocks of a ray block stay consecutive and in order
 within an XCD's share.  The shares cover the launch's tiles in order, each
 within one tile's work of an eighth of the total, and 8 equal slices give
-one slice per XCD."""
+one slice per XCD.  The pitch classes of the projector's angle blocks
+(fpl_pitch_classes) are checked for range, extent and determinism."""
 import os
 import shutil
 import subprocess
@@ -136,4 +137,64 @@ def test_dispatch_order_is_a_permutation_with_whole_ray_blocks(tmp_path):
     subprocess.run([cxx, "-O1", "-std=c++17", str(src), "-L", LIB, "-lacumen_hip", f"-Wl,-rpath,{LIB}", "-o", exe],
                    check=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
+
+
+PITCH_CHECKER = r"""
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+namespace acu {
+struct Error { std::string msg; };
+void fpl_pitch_classes(int N, int T, const float* theta_deg, int al, uint8_t* out);
+}
+int main() {
+  const int Ns[] = {3, 65, 513, 2049, 4097}, Ts[] = {1, 7, 180, 1500};
+  for (int N : Ns)
+    for (int T : Ts)
+      for (int al : {1, 2, 4}) {
+        std::vector<float> th(T);
+        for (int a = 0; a < T; ++a) th[a] = (float)(a * (180.0 / T));
+        const int nb = (T + 4 * al - 1) / (4 * al);
+        std::vector<uint8_t> a(nb + 1, 0xEE), b(nb + 1, 0xEE);
+        acu::fpl_pitch_classes(N, T, th.data(), al, a.data());
+        acu::fpl_pitch_classes(N, T, th.data(), al, b.data());
+        if (a[nb] != 0xEE) { printf("wrote past the blocks N=%d T=%d al=%d\n", N, T, al); return 1; }
+        for (int k = 0; k < nb; ++k)
+          if (a[k] > 7 || a[k] != b[k]) { printf("class N=%d T=%d al=%d k=%d: %d\n", N, T, al, k, a[k]); return 1; }
+      }
+  std::vector<float> th(8, 1.0f);
+  uint8_t out[8];
+  try {
+    acu::fpl_pitch_classes(513, 8, th.data(), 3, out);
+  } catch (const acu::Error&) {
+    printf("ok\n");
+    return 0;
+  }
+  printf("al = 3 accepted\n");
+  return 1;
+}
+"""
+
+
+def test_pitch_classes_in_range_deterministic_and_checked(tmp_path):
+    """fpl_pitch_classes (acumen_amd/csrc/fpl.hip), the host bank model that
+    picks a row pitch per angle block under the projector's lane map: one
+    class in 0..7 per block of 4*al angles, nothing written past the last
+    block, the same answer on a second call, and al outside {1, 2, 4}
+    rejected."""
+    so = os.path.join(LIB, "libacumen_hip.so")
+    if not os.path.exists(so):
+        pytest.fail("libacumen_hip.so not built (python -c 'import __graft_entry__ as g; g.build()')")
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "pitch_check.cpp"
+    src.write_text(PITCH_CHECKER)
+    exe = str(tmp_path / "pitch_check")
+    subprocess.run([cxx, "-O1", "-std=c++17", str(src), "-L", LIB, "-lacumen_hip", f"-Wl,-rpath,{LIB}", "-o", exe],
+                   check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
